@@ -1,0 +1,160 @@
+/*
+ * mbd_hip.h -- C ABI of libmbd_hip.so, the MI355X-native MBD@rsSCS engine.
+ *
+ * The reference program pmbd.x exposes no library API; its seams are the
+ * internal calls below, which these entry points replace one for one:
+ *
+ *   MBD_at_rsSCS(ene)            UTILS.f90:153-349   -> mbd_run
+ *   frequency loop + get_screened_alpha (per w)
+ *                                UTILS.f90:239-281,376-539 -> mbd_scs
+ *   get_mbd_rSCS(ene)            UTILS.f90:700-879   -> mbd_energy
+ *   get_vdw_param / get_beta / gauss_legendre_grid
+ *                                UTILS.f90:1324,353,1188 -> mbd_free_atom_params,
+ *                                mbd_beta, mbd_casimir_grid
+ *   module state set by the main program (geometry, lattice, settings)
+ *                                UTILS.f90:18-31, MBD_rsSCS.f90:66-135,211-212
+ *                                -> mbd_set_system
+ *
+ * Conventions: plain C types only; every host array is owned by the caller,
+ * contiguous, and read or written only during the call.  Coordinates are
+ * atom-major xyz[3*i + c] in bohr; the lattice is lat[3*k + c] = component c
+ * of lattice vector k, in bohr; cutoffs are in Angstrom, as in settings.in.
+ * Return codes mirror LAPACK's info: 0 ok, -i bad argument i, > 0 numerical
+ * failure (see each function), MBD_ERR_* for runtime errors.
+ * A handle is not re-entrant: one host thread per handle.
+ */
+#ifndef MBD_HIP_H
+#define MBD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mbd_handle mbd_handle;
+
+/* Solver forms. */
+#define MBD_MODE_FP64 0    /* plain FP64: tensors, Cholesky, eigen all FP64 */
+#define MBD_MODE_MIXED 1   /* FP32 tensors + FP32 MFMA Cholesky + FP64 refinement */
+#define MBD_MODE_LEARNED 2 /* MIXED tensors, network warm start + block-CG, LU fallback */
+
+#define MBD_ERR_HIP 1001     /* a HIP runtime call failed */
+#define MBD_ERR_NOSYS 1002   /* mbd_set_system not called */
+#define MBD_ERR_NOMEM 1003   /* device allocation failed */
+#define MBD_ERR_NONET 1004   /* MBD_MODE_LEARNED without network weights */
+
+/* Diagnostics of one call (all times in milliseconds of device time). */
+typedef struct {
+  int info_scs;       /* first nonzero SCS factorisation info (leading minor
+                         index, LAPACK style); the reference ignores it */
+  int fallback_used;  /* frequencies re-solved by the fallback factorisation */
+  int refine_iters;   /* max iterative-refinement / Krylov iterations */
+  int n_negative;     /* C-matrix eigenvalues < 0 (skipped, UTILS.f90:851) */
+  int info_eig;       /* 0 ok */
+  int supercell[3];   /* MBD supercell multiplicities (UTILS.f90:747-751) */
+  double residual;    /* max relative residual ||b - A x|| / ||b|| of the
+                         SCS solves (0 for the direct FP64 form) */
+  double t_scs_ms;
+  double t_mbd_ms;
+  double t_total_ms;
+} mbd_diag;
+
+/* Create a handle bound to HIP device `device` with its own stream. */
+int mbd_create(int device, mbd_handle **h);
+int mbd_destroy(mbd_handle *h);
+
+/* Last error message of the handle (empty string if none). */
+const char *mbd_last_error(const mbd_handle *h);
+
+/* Free-atom alpha0 [bohr^3], C6 [Ha bohr^6], R0 [bohr] for an element symbol
+ * (case-sensitive, <= 2 characters), as the REAL(4) literals of the
+ * reference table widened to double.  Returns 0, or -1 for an unknown
+ * element (the reference warns and STOPs, UTILS.f90:1693-1699). */
+int mbd_free_atom_params(const char *symbol, double *c6, double *alpha,
+                         double *r0);
+
+/* beta for an xc flag: 0.83 (PBE), 0.85 (PBE0/HSE), 1.0 otherwise. */
+double mbd_beta(int flag_xc);
+
+/* The 21 Casimir-Polder points (w_0 = 0) and weights. */
+void mbd_casimir_grid(double *omega21, double *weight21);
+
+/* Set the system.  xyz[3*n], hirshfeld[n], alpha0/c6_0/r0[n] (free-atom data
+ * already resolved), lattice[9] (ignored unless periodic), vacuum[3]
+ * (nonzero = vacuum axis), beta, cutoffs in Angstrom (SCS dipole, CFDM
+ * dipole, supercell radius).  Copies the inputs to the device. */
+int mbd_set_system(mbd_handle *h, int n_atoms, const double *xyz,
+                   const double *hirshfeld, const double *alpha0,
+                   const double *c6_0, const double *r0,
+                   const double *lattice, int periodic, const int *vacuum,
+                   double beta, double rc_scs, double rc_cfdm,
+                   double rc_supercell);
+
+/* SCS at all 21 grid points, then the C6 sums (UTILS.f90:239-315).
+ * Outputs (host, may be NULL): alpha_eff[n] (w = 0), coupled_pol[20*n]
+ * (element (k-1) + 20*i for grid point k = 1..20, the reference's
+ * coupled_atom_pol layout), molpol_eig[21*3] (ascending eigenvalues of the
+ * molecular 3x3 tensor per point), c6_eff[n], c6_mol.
+ * Returns 0 or MBD_ERR_*; the factorisation info is in d->info_scs. */
+int mbd_scs(mbd_handle *h, int mode, double *alpha_eff, double *coupled_pol,
+            double *molpol_eig, double *c6_eff, double *c6_mol, mbd_diag *d);
+
+/* E_MBD from alpha_eff[n] and c6_eff[n] (UTILS.f90:700-879): builds the
+ * coupled-QHO matrix over the supercell, all its eigenvalues in FP64, and
+ * E = 1/2 sum_{l>=0} sqrt(l) / Ncell - 3/2 sum w.  omega[n], reff[n] may be
+ * NULL.  Returns 0 or MBD_ERR_*; d->info_eig and d->n_negative report. */
+int mbd_energy(mbd_handle *h, const double *alpha_eff, const double *c6_eff,
+               double *e_mbd, double *omega, double *reff, mbd_diag *d);
+
+/* The whole MBD_at_rsSCS span on the device (the span pmbd.x times,
+ * MBD_rsSCS.f90:220-222): mbd_scs followed by mbd_energy without host round
+ * trips.  Output arrays as in mbd_scs / mbd_energy; any may be NULL. */
+int mbd_run(mbd_handle *h, int mode, double *e_mbd, double *alpha_eff,
+            double *coupled_pol, double *molpol_eig, double *c6_eff,
+            double *c6_mol, mbd_diag *d);
+
+/* ---- instrumentation (no counterpart in the reference; used by bench.py
+ * for the per-kernel roofline; the reference only times the whole span with
+ * MPI_Wtime, MBD_rsSCS.f90:220-224) ------------------------------------ */
+
+/* Enable (1) or disable (0) HIP-event timing of every kernel launch the
+ * handle's calls make, on the handle's stream.  Enabling clears totals. */
+int mbd_profile_enable(mbd_handle *h, int enable);
+
+/* Totals for kernel class `cls` since enabling: out[0] launches, out[1]
+ * device milliseconds, out[2] algorithmic bytes, out[3] algorithmic flops;
+ * *name receives the class name.  Returns 0, or -2 if cls is out of range. */
+int mbd_profile_read(mbd_handle *h, int cls, double *out, const char **name);
+
+/* ---- test hooks: device kernels on host arrays (parity tests) ---------- */
+
+/* The relay matrix A(i w_k) (full symmetric, column-major n x n, n = 3N)
+ * as the SCS builder writes it for grid point k; mode selects FP64 or FP32
+ * storage (returned widened).  Returns 0 or MBD_ERR_*. */
+int mbd_debug_relay_matrix(mbd_handle *h, int mode, int k, double *A);
+
+/* The coupled-QHO matrix for given alpha_eff, omega, reff (full symmetric,
+ * order 3*N*Ncell). */
+int mbd_debug_cfdm_matrix(mbd_handle *h, const double *alpha_eff,
+                          const double *omega, const double *reff,
+                          double *C);
+
+/* Lower Cholesky factor of `batch` symmetric matrices (n x n, column-major,
+ * consecutive) with the device factorisation; info[batch]. */
+int mbd_debug_cholesky(int device, int n, int batch, const double *A,
+                       double *L, int *info);
+
+/* All eigenvalues (ascending) of a symmetric n x n matrix (lower triangle
+ * read) with the device tridiagonalisation + bisection. */
+int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w);
+
+/* C = alpha A B^T + beta C with the MFMA GEMM (A m x k, B n x k, C m x n,
+ * column-major, leading dimensions = rows).  fp32 != 0 uses the FP32 MFMA
+ * kernel on inputs rounded to float (results widened). */
+int mbd_debug_gemm_nt(int device, int fp32, int m, int n, int k,
+                      double alpha, const double *A, const double *B,
+                      double beta, double *C);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,853 @@
+// C ABI of libmbd_hip.so: handle, system set-up, and the orchestration of
+// the SCS / C6 / MBD stages on one HIP stream (see include/mbd_hip.h for the
+// reference interface each entry point replaces).
+//
+// Device memory plan (3N = n, all column-major):
+//   relay matrices     nf_batch x n x n   (FP64, or FP32 + FP32 factor copy)
+//   inverted diagonal  nf_batch x ceil(n/NB) x NB x NB
+//   solutions          21 x n x 3 (FP64)
+//   C matrix           (n Ncell) x (n Ncell) FP64 + Householder workspace
+// Up to all 21 frequencies are processed per batch, bounded by a memory
+// budget (MBD_SCS_MEM_GB, default 64 GB) so one launch sequence factors every
+// frequency at once.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mbd_hip.h"
+#include "common.h"
+#include "kernels.h"
+#include "profile.h"
+#include "refine.h"
+
+using namespace mbd;
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    if (p) MBD_HIP(hipFree(p));
+    p = nullptr;
+    n = 0;
+    if (count == 0) return;
+    MBD_HIP(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct Elem {
+  const char *sym;
+  float alpha, c6, r0;
+};
+
+// Free-atom polarisability, C6 and vdW radius (UTILS.f90:1338-1691), kept as
+// float literals because the reference assigns REAL(4) literals.
+const Elem kElements[] = {
+    {"H", 4.5f, 6.5f, 3.1f},          {"He", 1.38f, 1.46f, 2.65f},
+    {"Li", 164.2f, 1387.0f, 4.16f},   {"Be", 38.0f, 214.0f, 4.17f},
+    {"B", 21.0f, 99.5f, 3.89f},       {"C", 12.0f, 46.6f, 3.59f},
+    {"N", 7.4f, 24.2f, 3.34f},        {"O", 5.4f, 15.6f, 3.19f},
+    {"F", 3.8f, 9.52f, 3.04f},        {"Ne", 2.67f, 6.38f, 2.91f},
+    {"Na", 162.7f, 1556.0f, 3.73f},   {"Mg", 71.0f, 627.0f, 4.27f},
+    {"Al", 60.0f, 528.0f, 4.33f},     {"Si", 37.0f, 305.0f, 4.2f},
+    {"P", 25.0f, 185.0f, 4.01f},      {"S", 19.6f, 134.0f, 3.86f},
+    {"Cl", 15.0f, 94.6f, 3.71f},      {"Ar", 11.1f, 64.3f, 3.55f},
+    {"K", 292.9f, 3897.0f, 3.71f},    {"Ca", 160.0f, 2221.0f, 4.65f},
+    {"Sc", 120.0f, 1383.0f, 4.59f},   {"Ti", 98.0f, 1044.0f, 4.51f},
+    {"V", 84.0f, 832.0f, 4.44f},      {"Cr", 78.0f, 602.0f, 3.99f},
+    {"Mn", 63.0f, 552.0f, 3.97f},     {"Fe", 56.0f, 482.0f, 4.23f},
+    {"Co", 50.0f, 408.0f, 4.18f},     {"Ni", 48.0f, 373.0f, 3.82f},
+    {"Cu", 42.0f, 253.0f, 3.76f},     {"Zn", 40.0f, 284.0f, 4.02f},
+    {"Ga", 60.0f, 498.0f, 4.19f},     {"Ge", 41.0f, 354.0f, 4.2f},
+    {"As", 29.0f, 246.0f, 4.11f},     {"Se", 25.0f, 210.0f, 4.04f},
+    {"Br", 20.0f, 162.0f, 3.93f},     {"Kr", 16.8f, 129.6f, 3.82f},
+    {"Rb", 319.2f, 4691.0f, 3.72f},   {"Sr", 199.0f, 3170.0f, 4.54f},
+    {"Y", 126.737f, 1968.58f, 4.8151f}, {"Zr", 119.97f, 1677.91f, 4.53f},
+    {"Nb", 101.603f, 1263.61f, 4.2365f}, {"Mo", 88.4225785f, 1028.73f, 4.099f},
+    {"Tc", 80.083f, 1390.87f, 4.076f}, {"Ru", 65.895f, 609.754f, 3.9953f},
+    {"Rh", 56.1f, 469.0f, 3.95f},     {"Pd", 23.68f, 157.5f, 3.66f},
+    {"Ag", 50.6f, 339.0f, 3.82f},     {"Cd", 39.7f, 452.0f, 3.99f},
+    {"In", 70.22f, 707.046f, 4.23198f}, {"Sn", 55.95f, 587.417f, 4.303f},
+    {"Sb", 43.67197f, 459.322f, 4.276f}, {"Te", 37.65f, 396.0f, 4.22f},
+    {"I", 35.0f, 385.0f, 4.17f},      {"Xe", 27.3f, 285.9f, 4.08f},
+    {"Cs", 427.12f, 6582.08f, 3.78f}, {"Ba", 275.0f, 5727.0f, 4.77f},
+    {"Hf", 99.52f, 1274.8f, 4.21f},   {"Ta", 82.53f, 1019.92f, 4.15f},
+    {"W", 71.041f, 847.93f, 4.08f},   {"Re", 63.04f, 710.2f, 4.02f},
+    {"Os", 55.055f, 596.67f, 3.84f},  {"Ir", 42.51f, 359.1f, 4.0f},
+    {"Pt", 39.68f, 347.1f, 3.92f},    {"Au", 36.5f, 298.0f, 3.86f},
+    {"Hg", 33.9f, 392.0f, 3.98f},     {"Tl", 69.92f, 717.44f, 3.91f},
+    {"Pb", 61.8f, 697.0f, 4.31f},     {"Bi", 49.02f, 571.0f, 4.32f},
+    {"Po", 45.013f, 530.92f, 4.097f}, {"At", 38.93f, 457.53f, 4.07f},
+    {"Rn", 33.54f, 390.63f, 4.23f},
+};
+
+// Casimir-Polder grid (UTILS.f90:1188-1211), REAL(4) literals.
+const float kOmega[21] = {0.0f,      0.0392901f, 0.118358f, 0.198912f, 0.282029f,
+                          0.368919f, 0.461006f,  0.560027f, 0.668179f, 0.788336f,
+                          0.92439f,  1.08179f,   1.26849f,  1.49661f,  1.78563f,
+                          2.16917f,  2.71062f,   3.54573f,  5.02734f,  8.44896f,
+                          25.4517f};
+const float kWeight[21] = {0.0f,      0.0786611f, 0.07964f,  0.0816475f, 0.0847872f,
+                           0.0892294f, 0.0952317f, 0.103172f, 0.113605f,  0.12735f,
+                           0.145652f, 0.170453f,  0.204917f, 0.254456f,  0.328962f,
+                           0.448092f, 0.655606f,  1.06596f,  2.06357f,   5.6851f,
+                           50.9558f};
+
+struct Timer {
+  hipEvent_t a, b;
+  Timer() {
+    MBD_HIP(hipEventCreate(&a));
+    MBD_HIP(hipEventCreate(&b));
+  }
+  ~Timer() {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+  void start(hipStream_t s) { MBD_HIP(hipEventRecord(a, s)); }
+  void stop(hipStream_t s) { MBD_HIP(hipEventRecord(b, s)); }
+  double ms() {
+    MBD_HIP(hipEventSynchronize(b));
+    float v = 0.f;
+    MBD_HIP(hipEventElapsedTime(&v, a, b));
+    return v;
+  }
+};
+
+// Lattice translations ((0-i) a1 + (0-j) a2) + (0-k) a3 over the box, i
+// outermost, in the reference's loop order (UTILS.f90:561-568).
+std::vector<double> image_box(const double *lat, const int *half) {
+  std::vector<double> t;
+  for (int i = -half[0]; i <= half[0]; ++i)
+    for (int j = -half[1]; j <= half[1]; ++j)
+      for (int k = -half[2]; k <= half[2]; ++k)
+        for (int c = 0; c < 3; ++c)
+          t.push_back(((double)(0 - i) * lat[c] + (double)(0 - j) * lat[3 + c]) +
+                      (double)(0 - k) * lat[6 + c]);
+  return t;
+}
+
+double norm3(const double *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+}  // namespace
+
+struct mbd_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool has_sys = false;
+  // system
+  int N = 0;
+  int periodic = 0;
+  double lat[9] = {0};
+  double latSL[9] = {0};
+  double beta = 1.0, rc_scs = 120.0, rc_cfdm = 100.0, rc_sc = 25.0;
+  int scs_img[3] = {0, 0, 0}, supercell[3] = {1, 1, 1}, cfdm_img[3] = {0, 0, 0};
+  int nimg_scs = 1, nimg_cfdm = 1, ncell = 1;
+  std::vector<double> a0_h, r0_h;
+  // device: system
+  DevBuf<double> xyz, hirsh, a0, c60, r0, omega, weight, img_scs, img_cfdm, lat_d;
+  DevBuf<int> cell;
+  // device: SCS
+  DevBuf<double> alpha_w, sgm, rvdw;
+  DevBuf<double> A64, inv64, X, alpha, P, eig3, c6mol, c6, R, D;
+  DevBuf<float> A32, L32, inv32;
+  DevBuf<int> info;
+  DevBuf<double> rnorm;
+  // device: MBD
+  DevBuf<double> C, om_at, reff, lam, e2, bounds, E, VW, tau, dd, ee, alph, pa, pb;
+  DevBuf<double> ext_alpha, ext_c6;
+  DevBuf<int> nneg;
+  // instrumentation
+  bool prof_on = false;
+  Profiler prof;
+
+  int n() const { return 3 * N; }
+};
+
+namespace {
+// Makes the handle's profiler the active one for the duration of a call.
+struct ProfGuard {
+  explicit ProfGuard(mbd_handle *h) { active_profiler() = h->prof_on ? &h->prof : nullptr; }
+  ~ProfGuard() { active_profiler() = nullptr; }
+};
+}  // namespace
+
+namespace {
+
+int fail(mbd_handle *h, const HipError &e) {
+  char buf[256];
+  snprintf(buf, sizeof buf, "HIP error %d (%s) at capi.hip:%d in %s", (int)e.err,
+           hipGetErrorString(e.err), e.line, e.what);
+  if (h) h->err = buf;
+  fprintf(stderr, "libmbd_hip: %s\n", buf);
+  return MBD_ERR_HIP;
+}
+
+size_t mem_budget() {
+  const char *s = getenv("MBD_SCS_MEM_GB");
+  double gb = s ? atof(s) : 64.0;
+  if (gb <= 0) gb = 64.0;
+  return (size_t)(gb * 1e9);
+}
+
+int freq_batch(const mbd_handle *h, size_t bytes_per_matrix) {
+  size_t b = mem_budget() / std::max<size_t>(bytes_per_matrix, 1);
+  return (int)std::max<size_t>(1, std::min<size_t>(kNumFreq, b));
+}
+
+// SCS at all 21 points into h->alpha (21 x N), h->P (21 x 9), then C6.
+void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
+  const int N = h->N, n = h->n();
+  hipStream_t s = h->stream;
+  h->alpha_w.ensure((size_t)kNumFreq * N);
+  h->sgm.ensure((size_t)kNumFreq * N);
+  h->rvdw.ensure(N);
+  atom_params(s, N, kNumFreq, h->omega.p, h->hirsh.p, h->a0.p, h->c60.p, h->r0.p,
+              h->alpha_w.p, h->sgm.p, h->rvdw.p);
+  const int lda = n;
+  const long sA = (long)n * lda;
+  const int nblk = ceil_div(n, kCholNB);
+  const long sinv = (long)nblk * kCholNB * kCholNB;
+  const long sX = 3L * n;
+  h->X.ensure((size_t)kNumFreq * sX);
+  h->alpha.ensure((size_t)kNumFreq * N);
+  h->P.ensure(9 * kNumFreq);
+  h->info.ensure(kNumFreq);
+  MBD_HIP(hipMemsetAsync(h->info.p, 0, sizeof(int) * kNumFreq, s));
+  const double rc = h->rc_scs / kBohr;
+  const int center = h->nimg_scs / 2;
+  int max_it = 0;
+  double max_res = 0.0;
+  if (mode == MBD_MODE_FP64) {
+    const int nb = freq_batch(h, (size_t)sA * 8 + (size_t)sinv * 8);
+    h->A64.ensure((size_t)nb * sA);
+    h->inv64.ensure((size_t)nb * sinv);
+    for (int f0 = 0; f0 < kNumFreq; f0 += nb) {
+      const int bs = std::min(nb, kNumFreq - f0);
+      scs_build<double>(s, N, h->xyz.p, h->nimg_scs, center, h->img_scs.p, rc, h->beta,
+                        h->alpha_w.p + (long)f0 * N, h->sgm.p + (long)f0 * N, h->rvdw.p,
+                        h->A64.p, lda, sA, bs);
+      potrf_batched<double>(s, n, h->A64.p, lda, sA, bs, h->inv64.p, sinv, h->info.p + f0);
+      set_rhs(s, n, h->X.p + f0 * sX, n, sX, bs);
+      potrs3_batched<double>(s, n, h->A64.p, lda, sA, bs, h->inv64.p, sinv,
+                             h->X.p + f0 * sX, n, sX);
+      contract(s, N, h->X.p + f0 * sX, n, sX, bs, h->alpha.p + (long)f0 * N, h->P.p + 9 * f0);
+    }
+  } else {
+    const int nb = freq_batch(h, (size_t)sA * 8 + (size_t)sinv * 4);
+    h->A32.ensure((size_t)nb * sA);
+    h->L32.ensure((size_t)nb * sA);
+    h->inv32.ensure((size_t)nb * sinv);
+    h->R.ensure((size_t)nb * sX);
+    h->rnorm.ensure((size_t)2 * kNumFreq);
+    for (int f0 = 0; f0 < kNumFreq; f0 += nb) {
+      const int bs = std::min(nb, kNumFreq - f0);
+      scs_build<float>(s, N, h->xyz.p, h->nimg_scs, center, h->img_scs.p, rc, h->beta,
+                       h->alpha_w.p + (long)f0 * N, h->sgm.p + (long)f0 * N, h->rvdw.p,
+                       h->A32.p, lda, sA, bs);
+      MBD_HIP(hipMemcpyAsync(h->L32.p, h->A32.p, sizeof(float) * (size_t)bs * sA,
+                             hipMemcpyDeviceToDevice, s));
+      potrf_batched<float>(s, n, h->L32.p, lda, sA, bs, h->inv32.p, sinv, h->info.p + f0);
+      double *Xb = h->X.p + f0 * sX;
+      set_rhs(s, n, Xb, n, sX, bs);
+      potrs3_batched<float>(s, n, h->L32.p, lda, sA, bs, h->inv32.p, sinv, Xb, n, sX);
+      // FP64 iterative refinement against the stored FP32 tensor
+      int it = 0;
+      double last_res = 0.0;
+      std::vector<double> rn(2 * bs);
+      for (; it < kRefineMaxIter; ++it) {
+        residual3<float>(s, n, h->A32.p, lda, sA, bs, Xb, n, sX, h->R.p, n, sX,
+                         h->rnorm.p);
+        MBD_HIP(hipMemcpyAsync(rn.data(), h->rnorm.p, sizeof(double) * 2 * bs,
+                               hipMemcpyDeviceToHost, s));
+        MBD_HIP(hipStreamSynchronize(s));
+        // ||b||_inf = 1 for the stacked identity, so max |r| is relative
+        double worst = 0.0;
+        for (int b = 0; b < bs; ++b) worst = std::max(worst, rn[2 * b]);
+        last_res = worst;
+        if (worst <= kRefineTol) break;
+        potrs3_batched<float>(s, n, h->L32.p, lda, sA, bs, h->inv32.p, sinv, h->R.p, n, sX);
+        axpy3(s, n, bs, h->R.p, n, sX, Xb, n, sX);
+      }
+      max_it = std::max(max_it, it);
+      max_res = std::max(max_res, last_res);
+      contract(s, N, Xb, n, sX, bs, h->alpha.p + (long)f0 * N, h->P.p + 9 * f0);
+    }
+  }
+  h->eig3.ensure(3 * kNumFreq);
+  h->c6mol.ensure(1);
+  h->c6.ensure(N);
+  c6_sums(s, N, h->alpha.p, h->P.p, h->weight.p, h->eig3.p, h->c6mol.p, h->c6.p);
+  std::vector<int> inf(kNumFreq);
+  MBD_HIP(hipMemcpyAsync(inf.data(), h->info.p, sizeof(int) * kNumFreq, hipMemcpyDeviceToHost, s));
+  MBD_HIP(hipStreamSynchronize(s));
+  if (d) {
+    d->info_scs = 0;
+    for (int k = 0; k < kNumFreq; ++k)
+      if (inf[k] != 0) {
+        d->info_scs = inf[k];
+        break;
+      }
+    d->refine_iters = max_it;
+    d->residual = max_res;
+  }
+}
+
+// E_MBD from device alpha_eff / c6_eff.
+void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
+                mbd_diag *d) {
+  const int N = h->N;
+  hipStream_t s = h->stream;
+  h->om_at.ensure(N);
+  h->reff.ensure(N);
+  mbd_params(s, N, alpha_eff_d, c6_d, h->a0.p, h->r0.p, h->om_at.p, h->reff.p);
+  const long nm = 3L * N * h->ncell;
+  h->C.ensure((size_t)nm * nm);
+  cfdm_build(s, N, h->ncell, h->xyz.p, h->cell.p, h->lat_d.p, h->nimg_cfdm,
+             h->nimg_cfdm / 2, h->img_cfdm.p, h->rc_cfdm / kBohr, h->beta,
+             alpha_eff_d, h->om_at.p, h->reff.p, h->C.p, nm);
+  TrdWork wk;
+  h->VW.ensure((size_t)nm * 3 * kTrdNB);
+  h->tau.ensure(nm);
+  h->dd.ensure(nm);
+  h->ee.ensure(nm);
+  h->alph.ensure(kTrdNB);
+  h->pa.ensure((size_t)(nm / 256 + 2) * (2 * kTrdNB + 1));
+  h->pb.ensure((size_t)(nm / 16 + 2));
+  wk.VW = h->VW.p;
+  wk.ldv = nm;
+  wk.tau = h->tau.p;
+  wk.d = h->dd.p;
+  wk.e = h->ee.p;
+  wk.alpha = h->alph.p;
+  wk.pa = h->pa.p;
+  wk.pb = h->pb.p;
+  tridiagonalize(s, (int)nm, h->C.p, nm, wk);
+  h->lam.ensure(nm);
+  h->e2.ensure(nm);
+  h->bounds.ensure(4);
+  tridiag_eigvals(s, (int)nm, h->dd.p, h->ee.p, h->e2.p, h->bounds.p, h->lam.p);
+  h->E.ensure(4);
+  h->nneg.ensure(1);
+  energy(s, (int)nm, h->lam.p, N, h->om_at.p, h->ncell, h->E.p, h->nneg.p);
+  if (d) {
+    d->info_eig = 0;
+    for (int k = 0; k < 3; ++k) d->supercell[k] = h->supercell[k];
+  }
+}
+
+void fetch(mbd_handle *h, double *dst, const double *src, size_t count) {
+  if (dst && count) MBD_HIP(hipMemcpyAsync(dst, src, sizeof(double) * count, hipMemcpyDeviceToHost, h->stream));
+}
+
+void fetch_scs_outputs(mbd_handle *h, double *alpha_eff, double *coupled_pol,
+                       double *molpol_eig, double *c6_eff, double *c6_mol) {
+  const int N = h->N;
+  fetch(h, alpha_eff, h->alpha.p, N);
+  if (coupled_pol) {
+    std::vector<double> a((size_t)kNumFreq * N);
+    fetch(h, a.data(), h->alpha.p, a.size());
+    MBD_HIP(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < N; ++i)
+      for (int k = 1; k < kNumFreq; ++k) coupled_pol[(k - 1) + 20L * i] = a[(size_t)k * N + i];
+  }
+  fetch(h, molpol_eig, h->eig3.p, 3 * kNumFreq);
+  fetch(h, c6_eff, h->c6.p, N);
+  fetch(h, c6_mol, h->c6mol.p, 1);
+  MBD_HIP(hipStreamSynchronize(h->stream));
+}
+
+bool valid_mode(int mode) { return mode == MBD_MODE_FP64 || mode == MBD_MODE_MIXED; }
+
+}  // namespace
+
+extern "C" {
+
+int mbd_create(int device, mbd_handle **out) {
+  if (!out) return -2;
+  mbd_handle *h = new mbd_handle;
+  h->device = device;
+  try {
+    MBD_HIP(hipSetDevice(device));
+    MBD_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->omega.ensure(kNumFreq);
+    h->weight.ensure(kNumFreq);
+    double w[kNumFreq], g[kNumFreq];
+    mbd_casimir_grid(w, g);
+    MBD_HIP(hipMemcpy(h->omega.p, w, sizeof w, hipMemcpyHostToDevice));
+    MBD_HIP(hipMemcpy(h->weight.p, g, sizeof g, hipMemcpyHostToDevice));
+  } catch (const HipError &e) {
+    int rc = fail(h, e);
+    delete h;
+    *out = nullptr;
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int mbd_destroy(mbd_handle *h) {
+  if (!h) return 0;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  DevBuf<double> *dbufs[] = {&h->xyz, &h->hirsh, &h->a0, &h->c60, &h->r0, &h->omega,
+                             &h->weight, &h->img_scs, &h->img_cfdm, &h->lat_d, &h->alpha_w,
+                             &h->sgm, &h->rvdw, &h->A64, &h->inv64, &h->X, &h->alpha,
+                             &h->P, &h->eig3, &h->c6mol, &h->c6, &h->R, &h->D, &h->rnorm,
+                             &h->C, &h->om_at, &h->reff, &h->lam, &h->e2, &h->bounds,
+                             &h->E, &h->VW, &h->tau, &h->dd, &h->ee, &h->alph, &h->pa,
+                             &h->pb, &h->ext_alpha, &h->ext_c6};
+  for (auto *b : dbufs) b->release();
+  h->A32.release();
+  h->L32.release();
+  h->inv32.release();
+  h->info.release();
+  h->cell.release();
+  h->nneg.release();
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+const char *mbd_last_error(const mbd_handle *h) { return h ? h->err.c_str() : ""; }
+
+int mbd_free_atom_params(const char *symbol, double *c6, double *alpha, double *r0) {
+  if (!symbol) return -1;
+  char s[3] = {0, 0, 0};
+  int k = 0;
+  for (const char *p = symbol; *p && k < 2; ++p)
+    if (*p != ' ') s[k++] = *p;
+  for (const auto &e : kElements)
+    if (strcmp(e.sym, s) == 0) {
+      if (c6) *c6 = (double)e.c6;
+      if (alpha) *alpha = (double)e.alpha;
+      if (r0) *r0 = (double)e.r0;
+      return 0;
+    }
+  return -1;
+}
+
+double mbd_beta(int flag_xc) {
+  switch (flag_xc) {
+    case 1: return (double)0.83f;
+    case 2:
+    case 3: return (double)0.85f;
+    default: return 1.0;
+  }
+}
+
+void mbd_casimir_grid(double *omega21, double *weight21) {
+  for (int k = 0; k < kNumFreq; ++k) {
+    if (omega21) omega21[k] = (double)kOmega[k];
+    if (weight21) weight21[k] = (double)kWeight[k];
+  }
+}
+
+int mbd_set_system(mbd_handle *h, int n_atoms, const double *xyz, const double *hirshfeld,
+                   const double *alpha0, const double *c6_0, const double *r0,
+                   const double *lattice, int periodic, const int *vacuum, double beta,
+                   double rc_scs, double rc_cfdm, double rc_supercell) {
+  if (!h) return -1;
+  if (n_atoms <= 0) return -2;
+  if (!xyz) return -3;
+  if (!hirshfeld) return -4;
+  if (!alpha0) return -5;
+  if (!c6_0) return -6;
+  if (!r0) return -7;
+  if (periodic && !lattice) return -8;
+  try {
+    MBD_HIP(hipSetDevice(h->device));
+    h->N = n_atoms;
+    h->periodic = periodic ? 1 : 0;
+    for (int c = 0; c < 9; ++c) h->lat[c] = periodic ? lattice[c] : 0.0;
+    h->beta = beta;
+    h->rc_scs = rc_scs;
+    h->rc_cfdm = rc_cfdm;
+    h->rc_sc = rc_supercell;
+    int vac[3] = {0, 0, 0};
+    if (vacuum)
+      for (int k = 0; k < 3; ++k) vac[k] = vacuum[k] ? 1 : 0;
+    // image ranges and supercell (UTILS.f90:1215-1320)
+    for (int k = 0; k < 3; ++k) {
+      h->scs_img[k] = 0;
+      h->supercell[k] = 1;
+      h->cfdm_img[k] = 0;
+      if (h->periodic && !vac[k]) {
+        const double la = norm3(h->lat + 3 * k);
+        h->scs_img[k] = (int)std::ceil((rc_scs / kBohr) / la);
+        h->supercell[k] = (int)std::ceil((rc_supercell / kBohr) / la);
+      }
+    }
+    for (int k = 0; k < 3; ++k)
+      for (int c = 0; c < 3; ++c) h->latSL[3 * k + c] = h->lat[3 * k + c] * h->supercell[k];
+    for (int k = 0; k < 3; ++k)
+      if (h->periodic && !vac[k])
+        h->cfdm_img[k] = (int)std::ceil((rc_cfdm / kBohr) / norm3(h->latSL + 3 * k));
+    std::vector<double> ti = image_box(h->lat, h->scs_img);
+    std::vector<double> tc = image_box(h->latSL, h->cfdm_img);
+    h->nimg_scs = (int)(ti.size() / 3);
+    h->nimg_cfdm = (int)(tc.size() / 3);
+    const int S1 = h->supercell[0], S2 = h->supercell[1], S3 = h->supercell[2];
+    h->ncell = S1 * S2 * S3;
+    std::vector<int> cells(3 * h->ncell);
+    for (int c1 = 0; c1 < S1; ++c1)
+      for (int c2 = 0; c2 < S2; ++c2)
+        for (int c3 = 0; c3 < S3; ++c3) {
+          const int a = c3 * S2 * S1 + c2 * S1 + c1;  // UTILS.f90:915
+          cells[3 * a] = c1;
+          cells[3 * a + 1] = c2;
+          cells[3 * a + 2] = c3;
+        }
+    auto up = [&](DevBuf<double> &b, const double *src, size_t cnt) {
+      b.ensure(cnt);
+      MBD_HIP(hipMemcpyAsync(b.p, src, sizeof(double) * cnt, hipMemcpyHostToDevice, h->stream));
+    };
+    up(h->xyz, xyz, 3L * n_atoms);
+    up(h->hirsh, hirshfeld, n_atoms);
+    up(h->a0, alpha0, n_atoms);
+    up(h->c60, c6_0, n_atoms);
+    up(h->r0, r0, n_atoms);
+    up(h->img_scs, ti.data(), ti.size());
+    up(h->img_cfdm, tc.data(), tc.size());
+    up(h->lat_d, h->lat, 9);
+    h->cell.ensure(cells.size());
+    MBD_HIP(hipMemcpyAsync(h->cell.p, cells.data(), sizeof(int) * cells.size(),
+                           hipMemcpyHostToDevice, h->stream));
+    MBD_HIP(hipStreamSynchronize(h->stream));
+    h->has_sys = true;
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_scs(mbd_handle *h, int mode, double *alpha_eff, double *coupled_pol,
+            double *molpol_eig, double *c6_eff, double *c6_mol, mbd_diag *d) {
+  if (!h) return -1;
+  if (!valid_mode(mode)) return -2;
+  if (!h->has_sys) return MBD_ERR_NOSYS;
+  try {
+    ProfGuard pg(h);
+    MBD_HIP(hipSetDevice(h->device));
+    Timer tm;
+    tm.start(h->stream);
+    run_scs(h, mode, d);
+    tm.stop(h->stream);
+    const double ms = tm.ms();
+    fetch_scs_outputs(h, alpha_eff, coupled_pol, molpol_eig, c6_eff, c6_mol);
+    if (d) {
+      d->t_scs_ms = ms;
+      d->t_total_ms = ms;
+    }
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_energy(mbd_handle *h, const double *alpha_eff, const double *c6_eff, double *e_mbd,
+               double *omega, double *reff, mbd_diag *d) {
+  if (!h) return -1;
+  if (!alpha_eff) return -2;
+  if (!c6_eff) return -3;
+  if (!e_mbd) return -4;
+  if (!h->has_sys) return MBD_ERR_NOSYS;
+  try {
+    ProfGuard pg(h);
+    MBD_HIP(hipSetDevice(h->device));
+    const int N = h->N;
+    h->ext_alpha.ensure(N);
+    h->ext_c6.ensure(N);
+    MBD_HIP(hipMemcpyAsync(h->ext_alpha.p, alpha_eff, sizeof(double) * N, hipMemcpyHostToDevice, h->stream));
+    MBD_HIP(hipMemcpyAsync(h->ext_c6.p, c6_eff, sizeof(double) * N, hipMemcpyHostToDevice, h->stream));
+    Timer tm;
+    tm.start(h->stream);
+    run_energy(h, h->ext_alpha.p, h->ext_c6.p, d);
+    tm.stop(h->stream);
+    const double ms = tm.ms();
+    double E[4];
+    int nn = 0;
+    fetch(h, E, h->E.p, 3);
+    MBD_HIP(hipMemcpyAsync(&nn, h->nneg.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    fetch(h, omega, h->om_at.p, N);
+    fetch(h, reff, h->reff.p, N);
+    MBD_HIP(hipStreamSynchronize(h->stream));
+    *e_mbd = E[0];
+    if (d) {
+      d->n_negative = nn;
+      d->t_mbd_ms = ms;
+      d->t_total_ms = ms;
+    }
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_run(mbd_handle *h, int mode, double *e_mbd, double *alpha_eff, double *coupled_pol,
+            double *molpol_eig, double *c6_eff, double *c6_mol, mbd_diag *d) {
+  if (!h) return -1;
+  if (!valid_mode(mode)) return -2;
+  if (!e_mbd) return -3;
+  if (!h->has_sys) return MBD_ERR_NOSYS;
+  try {
+    ProfGuard pg(h);
+    MBD_HIP(hipSetDevice(h->device));
+    Timer t_all, t_mbd;
+    t_all.start(h->stream);
+    run_scs(h, mode, d);
+    t_mbd.start(h->stream);
+    run_energy(h, h->alpha.p, h->c6.p, d);
+    t_mbd.stop(h->stream);
+    t_all.stop(h->stream);
+    const double ms_all = t_all.ms(), ms_mbd = t_mbd.ms();
+    double E[4];
+    int nn = 0;
+    fetch(h, E, h->E.p, 3);
+    MBD_HIP(hipMemcpyAsync(&nn, h->nneg.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    fetch_scs_outputs(h, alpha_eff, coupled_pol, molpol_eig, c6_eff, c6_mol);
+    *e_mbd = E[0];
+    if (d) {
+      d->n_negative = nn;
+      d->t_total_ms = ms_all;
+      d->t_mbd_ms = ms_mbd;
+      d->t_scs_ms = ms_all - ms_mbd;
+    }
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_profile_enable(mbd_handle *h, int enable) {
+  if (!h) return -1;
+  try {
+    MBD_HIP(hipSetDevice(h->device));
+    MBD_HIP(hipStreamSynchronize(h->stream));
+    h->prof.reset();
+    h->prof_on = enable != 0;
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_profile_read(mbd_handle *h, int cls, double *out, const char **name) {
+  if (!h) return -1;
+  if (cls < 0 || cls >= K_NUM_CLASSES) return -2;
+  try {
+    MBD_HIP(hipSetDevice(h->device));
+    h->prof.resolve();
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  if (out) {
+    out[0] = h->prof.count[cls];
+    out[1] = h->prof.ms[cls];
+    out[2] = h->prof.bytes[cls];
+    out[3] = h->prof.flops[cls];
+  }
+  if (name) *name = kernel_class_name(cls);
+  return 0;
+}
+
+// ---- test hooks ----------------------------------------------------------
+
+int mbd_debug_relay_matrix(mbd_handle *h, int mode, int k, double *A) {
+  if (!h) return -1;
+  if (!valid_mode(mode)) return -2;
+  if (k < 0 || k >= kNumFreq) return -3;
+  if (!A) return -4;
+  if (!h->has_sys) return MBD_ERR_NOSYS;
+  try {
+    MBD_HIP(hipSetDevice(h->device));
+    const int N = h->N, n = h->n();
+    hipStream_t s = h->stream;
+    h->alpha_w.ensure((size_t)kNumFreq * N);
+    h->sgm.ensure((size_t)kNumFreq * N);
+    h->rvdw.ensure(N);
+    atom_params(s, N, kNumFreq, h->omega.p, h->hirsh.p, h->a0.p, h->c60.p, h->r0.p,
+                h->alpha_w.p, h->sgm.p, h->rvdw.p);
+    const double rc = h->rc_scs / kBohr;
+    const size_t cnt = (size_t)n * n;
+    if (mode == MBD_MODE_FP64) {
+      h->A64.ensure(cnt);
+      scs_build<double>(s, N, h->xyz.p, h->nimg_scs, h->nimg_scs / 2, h->img_scs.p, rc,
+                        h->beta, h->alpha_w.p + (long)k * N, h->sgm.p + (long)k * N,
+                        h->rvdw.p, h->A64.p, n, (long)cnt, 1);
+      fetch(h, A, h->A64.p, cnt);
+      MBD_HIP(hipStreamSynchronize(s));
+    } else {
+      h->A32.ensure(cnt);
+      scs_build<float>(s, N, h->xyz.p, h->nimg_scs, h->nimg_scs / 2, h->img_scs.p, rc,
+                       h->beta, h->alpha_w.p + (long)k * N, h->sgm.p + (long)k * N,
+                       h->rvdw.p, h->A32.p, n, (long)cnt, 1);
+      std::vector<float> tmp(cnt);
+      MBD_HIP(hipMemcpyAsync(tmp.data(), h->A32.p, sizeof(float) * cnt, hipMemcpyDeviceToHost, s));
+      MBD_HIP(hipStreamSynchronize(s));
+      for (size_t q = 0; q < cnt; ++q) A[q] = tmp[q];
+    }
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_debug_cfdm_matrix(mbd_handle *h, const double *alpha_eff, const double *omega,
+                          const double *reff, double *C) {
+  if (!h) return -1;
+  if (!alpha_eff || !omega || !reff || !C) return -2;
+  if (!h->has_sys) return MBD_ERR_NOSYS;
+  try {
+    MBD_HIP(hipSetDevice(h->device));
+    const int N = h->N;
+    hipStream_t s = h->stream;
+    h->ext_alpha.ensure(N);
+    h->om_at.ensure(N);
+    h->reff.ensure(N);
+    MBD_HIP(hipMemcpyAsync(h->ext_alpha.p, alpha_eff, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    MBD_HIP(hipMemcpyAsync(h->om_at.p, omega, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    MBD_HIP(hipMemcpyAsync(h->reff.p, reff, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    const long nm = 3L * N * h->ncell;
+    h->C.ensure((size_t)nm * nm);
+    cfdm_build(s, N, h->ncell, h->xyz.p, h->cell.p, h->lat_d.p, h->nimg_cfdm, h->nimg_cfdm / 2,
+               h->img_cfdm.p, h->rc_cfdm / kBohr, h->beta, h->ext_alpha.p, h->om_at.p,
+               h->reff.p, h->C.p, nm);
+    fetch(h, C, h->C.p, (size_t)nm * nm);
+    MBD_HIP(hipStreamSynchronize(s));
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_debug_cholesky(int device, int n, int batch, const double *A, double *L, int *info) {
+  if (n <= 0) return -2;
+  if (batch <= 0) return -3;
+  if (!A) return -4;
+  if (!L) return -5;
+  if (!info) return -6;
+  double *dA = nullptr, *dinv = nullptr;
+  int *dinfo = nullptr;
+  try {
+    MBD_HIP(hipSetDevice(device));
+    const long sA = (long)n * n;
+    const int nblk = ceil_div(n, kCholNB);
+    const long sinv = (long)nblk * kCholNB * kCholNB;
+    MBD_HIP(hipMalloc(&dA, sizeof(double) * sA * batch));
+    MBD_HIP(hipMalloc(&dinv, sizeof(double) * sinv * batch));
+    MBD_HIP(hipMalloc(&dinfo, sizeof(int) * batch));
+    MBD_HIP(hipMemcpy(dA, A, sizeof(double) * sA * batch, hipMemcpyHostToDevice));
+    MBD_HIP(hipMemset(dinfo, 0, sizeof(int) * batch));
+    potrf_batched<double>(nullptr, n, dA, n, sA, batch, dinv, sinv, dinfo);
+    MBD_HIP(hipDeviceSynchronize());
+    MBD_HIP(hipMemcpy(L, dA, sizeof(double) * sA * batch, hipMemcpyDeviceToHost));
+    MBD_HIP(hipMemcpy(info, dinfo, sizeof(int) * batch, hipMemcpyDeviceToHost));
+  } catch (const HipError &e) {
+    if (dA) (void)hipFree(dA);
+    if (dinv) (void)hipFree(dinv);
+    if (dinfo) (void)hipFree(dinfo);
+    return fail(nullptr, e);
+  }
+  (void)hipFree(dA);
+  (void)hipFree(dinv);
+  (void)hipFree(dinfo);
+  return 0;
+}
+
+int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w) {
+  if (n <= 0) return -2;
+  if (!A) return -3;
+  if (!w) return -4;
+  std::vector<double *> bufs;
+  auto alloc = [&](size_t cnt) {
+    double *p = nullptr;
+    MBD_HIP(hipMalloc(&p, sizeof(double) * std::max<size_t>(cnt, 1)));
+    bufs.push_back(p);
+    return p;
+  };
+  try {
+    MBD_HIP(hipSetDevice(device));
+    // full symmetric copy from the lower triangle
+    std::vector<double> full((size_t)n * n);
+    for (int c = 0; c < n; ++c)
+      for (int r = c; r < n; ++r) full[r + (size_t)c * n] = full[c + (size_t)r * n] = A[r + (size_t)c * n];
+    double *dA = alloc((size_t)n * n);
+    MBD_HIP(hipMemcpy(dA, full.data(), sizeof(double) * full.size(), hipMemcpyHostToDevice));
+    TrdWork wk;
+    wk.VW = alloc((size_t)n * 3 * kTrdNB);
+    wk.ldv = n;
+    wk.tau = alloc(n);
+    wk.d = alloc(n);
+    wk.e = alloc(n);
+    wk.alpha = alloc(kTrdNB);
+    wk.pa = alloc((size_t)(n / 256 + 2) * (2 * kTrdNB + 1));
+    wk.pb = alloc((size_t)(n / 16 + 2));
+    tridiagonalize(nullptr, n, dA, n, wk);
+    double *e2 = alloc(n), *bounds = alloc(4), *lam = alloc(n);
+    tridiag_eigvals(nullptr, n, wk.d, wk.e, e2, bounds, lam);
+    MBD_HIP(hipDeviceSynchronize());
+    MBD_HIP(hipMemcpy(w, lam, sizeof(double) * n, hipMemcpyDeviceToHost));
+  } catch (const HipError &e) {
+    for (auto *p : bufs) (void)hipFree(p);
+    return fail(nullptr, e);
+  }
+  for (auto *p : bufs) (void)hipFree(p);
+  return 0;
+}
+
+int mbd_debug_gemm_nt(int device, int fp32, int m, int n, int k, double alpha, const double *A,
+                      const double *B, double beta, double *C) {
+  if (m <= 0 || n <= 0 || k <= 0) return -3;
+  if (!A || !B || !C) return -7;
+  void *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  try {
+    MBD_HIP(hipSetDevice(device));
+    const size_t es = fp32 ? 4 : 8;
+    MBD_HIP(hipMalloc(&dA, es * m * k));
+    MBD_HIP(hipMalloc(&dB, es * n * k));
+    MBD_HIP(hipMalloc(&dC, es * m * n));
+    if (fp32) {
+      std::vector<float> a(A, A + (size_t)m * k), b(B, B + (size_t)n * k), c(C, C + (size_t)m * n);
+      MBD_HIP(hipMemcpy(dA, a.data(), 4 * a.size(), hipMemcpyHostToDevice));
+      MBD_HIP(hipMemcpy(dB, b.data(), 4 * b.size(), hipMemcpyHostToDevice));
+      MBD_HIP(hipMemcpy(dC, c.data(), 4 * c.size(), hipMemcpyHostToDevice));
+      gemm_nt(nullptr, false, m, n, k, (float)alpha, (const float *)dA, m, 0, (const float *)dB,
+              n, 0, (float)beta, (float *)dC, m, 0, 1);
+      MBD_HIP(hipDeviceSynchronize());
+      MBD_HIP(hipMemcpy(c.data(), dC, 4 * c.size(), hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < c.size(); ++q) C[q] = c[q];
+    } else {
+      MBD_HIP(hipMemcpy(dA, A, 8 * (size_t)m * k, hipMemcpyHostToDevice));
+      MBD_HIP(hipMemcpy(dB, B, 8 * (size_t)n * k, hipMemcpyHostToDevice));
+      MBD_HIP(hipMemcpy(dC, C, 8 * (size_t)m * n, hipMemcpyHostToDevice));
+      gemm_nt(nullptr, false, m, n, k, alpha, (const double *)dA, m, 0, (const double *)dB, n, 0,
+              beta, (double *)dC, m, 0, 1);
+      MBD_HIP(hipDeviceSynchronize());
+      MBD_HIP(hipMemcpy(C, dC, 8 * (size_t)m * n, hipMemcpyDeviceToHost));
+    }
+  } catch (const HipError &e) {
+    if (dA) (void)hipFree(dA);
+    if (dB) (void)hipFree(dB);
+    if (dC) (void)hipFree(dC);
+    return fail(nullptr, e);
+  }
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dC);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+// Shared definitions of the MBD@rsSCS HIP kernels (gfx950 / CDNA4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace mbd {
+
+constexpr double kBohr = 0.52917721;                // UTILS.f90:85
+constexpr double kPi = 3.14159265358979323846;      // UTILS.f90:86
+constexpr double kOtd = 0.33333333333333333333;     // UTILS.f90:87
+constexpr double kThreeOverPi = 0.954929658551372;  // UTILS.f90:288,310,313
+constexpr int kNumFreq = 21;                         // w_0 = 0 plus 20 points
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct HipError {
+  hipError_t err;
+  const char *what;
+  int line;
+};
+
+#define MBD_HIP(call)                                              \
+  do {                                                             \
+    hipError_t e_ = (call);                                        \
+    if (e_ != hipSuccess) throw ::mbd::HipError{e_, #call, __LINE__}; \
+  } while (0)
+
+#define MBD_LAUNCH_CHECK() MBD_HIP(hipGetLastError())
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Column-major strided batch of matrices.
+template <typename T>
+struct MatBatch {
+  T *p;
+  int ld;
+  long stride;
+  __host__ __device__ T *at(int b) const { return p + (long)b * stride; }
+};
+
+// ---- launchers implemented in the .hip files -----------------------------
+
+// C[m x n] = alpha * A[m x k] * B[n x k]^T + beta * C, batched over
+// blockIdx.z; lower != 0 restricts to tiles that touch the lower triangle
+// (row tile >= column tile), for symmetric rank-k updates.
+void gemm_nt(hipStream_t s, bool lower, int m, int n, int k, double alpha,
+             const double *A, int lda, long sA, const double *B, int ldb,
+             long sB, double beta, double *C, int ldc, long sC, int batch);
+void gemm_nt(hipStream_t s, bool lower, int m, int n, int k, float alpha,
+             const float *A, int lda, long sA, const float *B, int ldb,
+             long sB, float beta, float *C, int ldc, long sC, int batch);
+
+}  // namespace mbd

@@ -1,0 +1,179 @@
+// MFMA GEMM "NT" update for CDNA4: C = alpha * A * B^T + beta * C.
+//
+// This is the trailing-update engine of the blocked factorisations: the
+// Cholesky panel solve (as a product with the inverted diagonal block) and
+// symmetric rank-k update, and the two-sided Householder SYR2K of the
+// tridiagonalisation.  One 256-thread workgroup owns a 128 x 128 tile of C;
+// its four wave64s each own a 64 x 64 sub-tile held as 4 x 4 MFMA 16x16
+// accumulators.  K is staged through double-buffered LDS in steps of 16 with
+// the next global tile prefetched into registers while the MFMAs run.
+//
+// FP64 uses v_mfma_f64_16x16x4_f64, FP32 uses v_mfma_f32_16x16x4_f32; both
+// take one A and one B element per lane per K=4 step:
+//   A operand: lane l -> A[row l%16][k l/16]
+//   B operand: lane l -> B^T[k l/16][col l%16]
+//   D (FP32):  lane l -> D[row 4*(l/16)+v][col l%16], v = 0..3
+//   D (FP64):  lane l -> D[row (l/16)+4*v][col l%16], v = 0..3
+#include "common.h"
+
+namespace mbd {
+
+namespace {
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  using acc_t = v4d;
+  // row of accumulator element v held by lane group g = lane / 16
+  __device__ static int row(int g, int v) { return g + 4 * v; }
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  using acc_t = v4f;
+  __device__ static int row(int g, int v) { return 4 * g + v; }
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+constexpr int PER_T = BM * BK / NT;  // elements of each operand per thread
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(NT) void k_gemm_nt(
+    int m, int n, int k, T alpha, const T *__restrict__ A, int lda, long sA,
+    const T *__restrict__ B, int ldb, long sB, T beta, T *__restrict__ C,
+    int ldc, long sC, int tiles_n) {
+  int tm, tn;
+  const int idx = blockIdx.x;
+  if (LOWER) {
+    tm = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+    while ((tm + 1) * (tm + 2) / 2 <= idx) ++tm;
+    while (tm * (tm + 1) / 2 > idx) --tm;
+    tn = idx - tm * (tm + 1) / 2;
+  } else {
+    tm = idx / tiles_n;
+    tn = idx - tm * tiles_n;
+  }
+  const long bz = blockIdx.z;
+  A += bz * sA;
+  B += bz * sB;
+  C += bz * sC;
+  const int row0 = tm * BM, col0 = tn * BN;
+
+  __shared__ T As[2][BK][BM];
+  __shared__ T Bs[2][BK][BN];
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  typename Mfma<T>::acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+
+  T ra[PER_T], rb[PER_T];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < PER_T; ++i) {
+      const int e = t + NT * i;
+      const int r = e & (BM - 1), kk = e / BM;
+      const int gk = k0 + kk;
+      const int gr = row0 + r, gc = col0 + r;
+      ra[i] = (gr < m && gk < k) ? A[gr + (long)gk * lda] : T(0);
+      rb[i] = (gc < n && gk < k) ? B[gc + (long)gk * ldb] : T(0);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < PER_T; ++i) {
+      const int e = t + NT * i;
+      const int r = e & (BM - 1), kk = e / BM;
+      As[buf][kk][r] = ra[i];
+      Bs[buf][kk][r] = rb[i];
+    }
+  };
+
+  const int nk = (k + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[cur][kk + lk][wm * 64 + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = Bs[cur][kk + lk][wn * 64 + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
+    }
+    if (kt + 1 < nk) sstore(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = row0 + wm * 64 + i * 16 + Mfma<T>::row(lk, v);
+        const int c = col0 + wn * 64 + j * 16 + lr;
+        if (r < m && c < n && (!LOWER || r >= c)) {
+          T *pc = C + r + (long)c * ldc;
+          T val = alpha * acc[i][j][v];
+          if (beta != T(0)) val += beta * (*pc);
+          *pc = val;
+        }
+      }
+}
+
+template <typename T>
+void launch(hipStream_t s, bool lower, int m, int n, int k, T alpha,
+            const T *A, int lda, long sA, const T *B, int ldb, long sB,
+            T beta, T *C, int ldc, long sC, int batch) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  const int tmn = ceil_div(m, BM), tnn = ceil_div(n, BN);
+  if (lower) {
+    // symmetric update: m == n, only tiles with row tile >= column tile
+    const int tiles = tmn * (tmn + 1) / 2;
+    dim3 grid(tiles, 1, batch);
+    hipLaunchKernelGGL((k_gemm_nt<T, true>), grid, dim3(NT), 0, s, m, n, k,
+                       alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, tnn);
+  } else {
+    dim3 grid(tmn * tnn, 1, batch);
+    hipLaunchKernelGGL((k_gemm_nt<T, false>), grid, dim3(NT), 0, s, m, n, k,
+                       alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, tnn);
+  }
+  MBD_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void gemm_nt(hipStream_t s, bool lower, int m, int n, int k, double alpha,
+             const double *A, int lda, long sA, const double *B, int ldb,
+             long sB, double beta, double *C, int ldc, long sC, int batch) {
+  launch<double>(s, lower, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C,
+                 ldc, sC, batch);
+}
+
+void gemm_nt(hipStream_t s, bool lower, int m, int n, int k, float alpha,
+             const float *A, int lda, long sA, const float *B, int ldb,
+             long sB, float beta, float *C, int ldc, long sC, int batch) {
+  launch<float>(s, lower, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C,
+                ldc, sC, batch);
+}
+
+}  // namespace mbd

@@ -1,0 +1,61 @@
+// Host-side launchers of the MBD@rsSCS kernels (internal to libmbd_hip.so).
+#pragma once
+
+#include "common.h"
+
+namespace mbd {
+
+constexpr int kCholNB = 128;   // Cholesky block size (diagonal block in LDS)
+constexpr int kTrdNB = 64;     // Householder panel width
+constexpr int kTrdTail = 128;  // trailing order reduced by one workgroup
+
+// build.hip
+void atom_params(hipStream_t s, int N, int nf, const double *omega,
+                 const double *hirsh, const double *a0, const double *c60,
+                 const double *r0, double *alpha_w, double *sgm, double *rvdw);
+template <typename T>
+void scs_build(hipStream_t s, int N, const double *xyz, int nimg, int center,
+               const double *img, double rc_bohr, double beta,
+               const double *alpha_w, const double *sgm, const double *rvdw,
+               T *A, int lda, long stride, int nf);
+void mbd_params(hipStream_t s, int N, const double *alpha_eff,
+                const double *c6_eff, const double *a0, const double *r0,
+                double *omega, double *reff);
+void cfdm_build(hipStream_t s, int N, int ncell, const double *xyz,
+                const int *cell, const double *lat, int nimg, int center,
+                const double *img, double rc_bohr, double beta,
+                const double *alpha_eff, const double *omega,
+                const double *reff, double *C, long ldc);
+
+// chol.hip
+template <typename T>
+void potrf_batched(hipStream_t s, int n, T *A, int lda, long sA, int batch,
+                   T *inv, long sinv_blk, int *info);
+template <typename T>
+void potrs3_batched(hipStream_t s, int n, const T *L, int lda, long sA,
+                    int batch, const T *inv, long sinv_blk, double *X, int ldx,
+                    long sX);
+void set_rhs(hipStream_t s, int n, double *X, int ldx, long sX, int batch);
+void contract(hipStream_t s, int N, const double *X, int ldx, long sX,
+              int batch, double *alpha, double *P);
+void c6_sums(hipStream_t s, int N, const double *alpha, const double *P,
+             const double *weight, double *eig, double *c6_mol, double *c6);
+
+// eig.hip
+struct TrdWork {
+  double *VW;     // n x 3*kTrdNB: [V | W | V]
+  long ldv;
+  double *tau;    // n
+  double *d;      // n
+  double *e;      // n
+  double *alpha;  // kTrdNB
+  double *pa;     // partial sums of colupd, (n/256+1) x (2 kTrdNB + 1)
+  double *pb;     // partial dots of gemv, n/16 + 1
+};
+void tridiagonalize(hipStream_t s, int n, double *A, long lda, TrdWork &wk);
+void tridiag_eigvals(hipStream_t s, int n, const double *d, const double *e,
+                     double *e2, double *bounds, double *w);
+void energy(hipStream_t s, int n, const double *lam, int N,
+            const double *omega, int ncell, double *out, int *nneg);
+
+}  // namespace mbd

@@ -642,9 +642,9 @@ static int or_scs_one(int N, const double *xyz, const double *hirsh,
                       double beta, double w, double *A, double *X,
                       double *alpha_scs, double *P) {
   const int n = 3 * N;
-  double *aw = (double *)malloc(sizeof(double) * (size_t)N);
-  double *sg = (double *)malloc(sizeof(double) * (size_t)N);
-  double *rv = (double *)malloc(sizeof(double) * (size_t)N);
+  double *aw = (double *)calloc((size_t)N + 1, sizeof(double));
+  double *sg = (double *)calloc((size_t)N + 1, sizeof(double));
+  double *rv = (double *)calloc((size_t)N + 1, sizeof(double));
   for (int i = 0; i < N; ++i) { /* UTILS.f90:251-255 */
     or_alpha_sigma(hirsh[i], c60[i], a0[i], w, aw + i, sg + i);
     rv[i] = pow(hirsh[i], OR_OTD) * r0[i];

@@ -53,6 +53,23 @@ def main(ref):
         golden[name] = r
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(golden, f, indent=1, sort_keys=True)
+    # the free-atom table as the reference's decimal literals
+    # (SELECT CASE in get_vdw_param): {symbol: [alpha, C6, R0]}
+    table, cur = {}, None
+    for line in open(os.path.join(ref, "UTILS.f90")):
+        if re.match(r"\s*case default", line):
+            cur = None
+            continue
+        m = re.match(r"\s*case\('(\w+)'\)", line)
+        if m:
+            cur = m.group(1)
+            table[cur] = [None, None, None]
+            continue
+        m = re.match(r"\s*(alpha|C6|R0)=([0-9.]+)\s*$", line)
+        if m and cur is not None:
+            table[cur][["alpha", "C6", "R0"].index(m.group(1))] = float(m.group(2))
+    with open(os.path.join(HERE, "vdw_table.json"), "w") as f:
+        json.dump(table, f, indent=0)
 
 
 if __name__ == "__main__":

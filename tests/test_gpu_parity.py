@@ -1,0 +1,167 @@
+"""The HIP path against the CPU oracle on the same inputs, through the C ABI.
+
+FP64 form: relay and coupled-QHO matrices to ~1e-13 relative, alpha_SCS and C6
+to 1e-10 relative, |dE_MBD| <= 1e-8 Ha, and the three
+reference goldens to their printed precision.  Mixed form: |dE| <= 1 uHa,
+C6 within 1e-5 relative.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from mbd_amd import generators, io
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine():
+    from mbd_amd.engine import MbdEngine
+    return MbdEngine(0)
+
+
+def _load(name, settings=None):
+    geo = io.read_geometry(os.path.join(GOLDEN, name))
+    st = io.read_settings(os.path.join(GOLDEN, settings) if settings else None)
+    return geo, st
+
+
+def _small_periodic():
+    # 64 atoms in a 10 A cube, 12 A SCS/supercell cutoffs: 2x2x2 supercell
+    return generators.random_periodic(64, 10.0, seed=11, rc_scs=12.0, rc_sc=12.0, rc_cfdm=30.0)
+
+
+def _tiochf():
+    # the reference's TiOCHF slab with z as vacuum axis and a 6 A supercell
+    # radius; its relay matrices are SPD but one O atom screens to a negative
+    # alpha, so only the matrix builders are compared on it
+    geo, st = _load("TiOCHF.in", "settings.in")
+    st.mbd_supercell_cutoff = 6.0
+    st.mbd_scs_vacuum_axis = (False, False, True)
+    return geo, st
+
+
+def _small_sheet():
+    # 6 x 4 hexagonal sheet (48 atoms) with vacuum axis z: 2 x 3 x 1 supercell
+    return generators.hex_sheet(6, 4, seed=3, rc_scs=30.0)
+
+
+CASES = {
+    "C6H6": lambda: _load("C6H6.in"),
+    "H2O": lambda: _load("H2O.in"),
+    "graphene_111": lambda: _load("graphene_111.in", "settings.in"),
+    "TiOCHF_slab": _tiochf,
+    "sheet48_vacuum_z": _small_sheet,
+    "random64_periodic": _small_periodic,
+    "ring": generators.ring,
+}
+
+
+def _oracle_system(geo, st):
+    xyz, lat = geo.to_bohr()
+    return O.system_from(geo.names, xyz, geo.hirshfeld, lat, geo.periodic, st)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = _engine()
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("case", ["C6H6", "graphene_111", "random64_periodic", "TiOCHF_slab"])
+def test_relay_matrix_fp64(eng, case):
+    geo, st = CASES[case]()
+    eng.set_geometry(geo, st)
+    sy = _oracle_system(geo, st)
+    for k in (0, 7, 20):
+        A = eng.relay_matrix(k)
+        want = O.relay_matrix(sy, k)
+        scale = np.abs(want).max()
+        assert np.allclose(A, want, rtol=1e-12, atol=1e-14 * scale), (case, k)
+        assert np.array_equal(A, A.T)
+
+
+def test_relay_matrix_fp32(eng):
+    geo, st = CASES["graphene_111"]()
+    eng.set_geometry(geo, st)
+    sy = _oracle_system(geo, st)
+    A = eng.relay_matrix(5, mode="mixed")
+    want = O.relay_matrix(sy, 5)
+    assert np.allclose(A, want, rtol=2e-6, atol=2e-7 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("case", ["graphene_111", "random64_periodic"])
+def test_cfdm_matrix(eng, case):
+    geo, st = CASES[case]()
+    eng.set_geometry(geo, st)
+    sy = _oracle_system(geo, st)
+    r = O.mbd_run(sy)
+    nc = int(np.prod(r.supercell))
+    C = eng.cfdm_matrix(r.alpha_eff, r.omega, r.reff, nc)
+    want = O.cfdm_matrix(sy, r.alpha_eff, r.omega, r.reff)
+    assert C.shape == want.shape
+    assert np.allclose(C, want, rtol=1e-12, atol=1e-14 * np.abs(want).max())
+
+
+FULL = [c for c in CASES if c != "TiOCHF_slab"]
+
+
+@pytest.mark.parametrize("case", FULL)
+def test_full_run_fp64(eng, case):
+    geo, st = CASES[case]()
+    eng.set_geometry(geo, st)
+    got = eng.run("fp64")
+    want = O.mbd_run(_oracle_system(geo, st))
+    print(case, "E gpu", got.e_mbd, "oracle", want.e_mbd, "diag", got.diag)
+    assert got.diag["info_scs"] == 0
+    assert abs(got.e_mbd - want.e_mbd) <= 1e-8
+    assert np.allclose(got.alpha_eff, want.alpha_eff, rtol=1e-10)
+    assert np.allclose(got.coupled_pol, want.coupled_pol, rtol=1e-10)
+    assert np.allclose(got.c6_eff, want.c6_eff, rtol=1e-10)
+    assert got.c6_mol == pytest.approx(want.c6_mol, rel=1e-10)
+    assert np.allclose(got.molpol_eig, want.molpol_eig, rtol=1e-10)
+    assert got.diag["supercell"] == list(want.supercell)
+    assert got.diag["n_negative"] == want.nimag
+
+
+@pytest.mark.parametrize("name", ["C6H6", "graphene_111", "PBBA"])
+def test_goldens_fp64(eng, name):
+    with open(os.path.join(GOLDEN, "golden.json")) as f:
+        g = json.load(f)[name]
+    geo, st = _load(g["geometry"], g["settings"])
+    eng.set_geometry(geo, st)
+    r = eng.run("fp64")
+    assert abs(r.e_mbd - g["e_mbd_ha"]) <= 6e-10
+    assert abs(r.c6_mol - g["c6_mol"]) <= 6e-9
+    for idx, (_, c6, alpha) in g["atoms"].items():
+        i = int(idx) - 1
+        assert abs(r.c6_eff[i] - c6) <= 6e-7
+        assert abs(r.alpha_eff[i] - alpha) <= 6e-7
+
+
+@pytest.mark.parametrize("case", ["C6H6", "graphene_111", "random64_periodic",
+                                  "sheet48_vacuum_z"])
+def test_full_run_mixed(eng, case):
+    geo, st = CASES[case]()
+    eng.set_geometry(geo, st)
+    got = eng.run("mixed")
+    want = O.mbd_run(_oracle_system(geo, st))
+    print(case, "mixed dE", got.e_mbd - want.e_mbd, "diag", got.diag)
+    assert abs(got.e_mbd - want.e_mbd) <= 1e-6
+    assert np.allclose(got.c6_eff, want.c6_eff, rtol=1e-5)
+    assert got.c6_mol == pytest.approx(want.c6_mol, rel=1e-5)
+    assert np.allclose(got.alpha_eff, want.alpha_eff, rtol=1e-5)
+    assert got.diag["residual"] <= 1e-12
+
+
+def test_run_is_deterministic(eng):
+    geo, st = CASES["random64_periodic"]()
+    eng.set_geometry(geo, st)
+    a = eng.run("fp64")
+    b = eng.run("fp64")
+    assert a.e_mbd == b.e_mbd
+    assert np.array_equal(a.alpha_eff, b.alpha_eff)
