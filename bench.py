@@ -7,9 +7,12 @@ omega sums, coupled-QHO matrix, all eigenvalues, energy.  The workload is the
 cutoff 40 A, supercell 25 A, CFDM 100 A), so n_SCS = n_MBD = 12288.  The
 geometry is resident on the device before the timed region.
 
-Headline form: mixed precision (FP32 tensors + FP32 MFMA Cholesky with FP64
-refinement; FP64 eigen stage).  The plain FP64 form is timed in the same run,
-and the mixed form's errors against it are reported.
+Forms timed in the same run: plain FP64 (FP64 tensors, FP64 MFMA Cholesky);
+mixed (FP32 tensors, FP32 MFMA Cholesky with FP64 refinement); learned
+(FP32 tensors, batched block-PCG over all 21 frequencies with the network's
+warm start when weights are loaded, gated on the FP64 residual with Cholesky
+fallback).  The eigen stage is FP64 in every form.  The headline value is the
+learned form; each form's errors against the FP64 form are reported.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -90,6 +93,7 @@ def main():
     ap.add_argument("--config", default="c3_random_4096")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--headline", default="learned", choices=["mixed", "learned"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding "
@@ -120,40 +124,57 @@ def main():
                 "n_negative": r64.diag["n_negative"]}
         log(f"bench: fp64 {dt64 / args.steps:.3f} s/solve E={r64.e_mbd:.10f}")
 
-    # headline: mixed form, with live per-kernel event timing
-    dtm, em, dm = timed_solves(eng, "mixed", args.steps, args.warmup)
-    eng.profile_enable(True)
-    rm = eng.run("mixed")
-    prof_m = eng.profile_read()
-    eng.profile_enable(False)
-    log(f"bench: mixed {dtm / args.steps:.3f} s/solve E={rm.e_mbd:.10f}")
-    out["value"] = args.steps / dtm
-    out["ms_per_step"] = 1e3 * dtm / args.steps
-    out["s_per_solve"] = dtm / args.steps
+    def errors(r, ref):
+        return {"e_mbd_ha": abs(r.e_mbd - ref.e_mbd),
+                "c6_rel_max": float(np.max(np.abs(r.c6_eff / ref.c6_eff - 1.0))),
+                "alpha_scs_rel_max": float(np.max(np.abs(r.coupled_pol / ref.coupled_pol - 1.0))),
+                "c6_mol_rel": abs(r.c6_mol / ref.c6_mol - 1.0)}
+
+    forms = {}
+    for form in ("mixed", "learned"):
+        dt, e, dg = timed_solves(eng, form, args.steps, args.warmup)
+        eng.profile_enable(True)
+        r = eng.run(form)
+        prof = eng.profile_read()
+        eng.profile_enable(False)
+        forms[form] = {"dt": dt, "r": r, "prof": prof}
+        log(f"bench: {form} {dt / args.steps:.3f} s/solve E={r.e_mbd:.10f} diag={r.diag}")
+    head = args.headline
+    hf = forms[head]
+    rm, prof_m = hf["r"], hf["prof"]
+    out["value"] = args.steps / hf["dt"]
+    out["ms_per_step"] = 1e3 * hf["dt"] / args.steps
+    out["s_per_solve"] = hf["dt"] / args.steps
     out["config"] = {"workload": f"{args.config}: {geo.n_atoms}-atom periodic cell, "
                                  f"n_SCS={n}, n_MBD={n * int(np.prod(rm.diag['supercell']))}",
-                     "form": "mixed", "scs_cutoff_A": st.mbd_scs_dip_cutoff,
+                     "form": head, "scs_cutoff_A": st.mbd_scs_dip_cutoff,
                      "supercell_cutoff_A": st.mbd_supercell_cutoff,
                      "cfdm_cutoff_A": st.mbd_cfdm_dip_cutoff}
-    out["refine_iters"] = rm.diag["refine_iters"]
-    out["residual"] = rm.diag["residual"]
+    out["metric"] = f"MBD@rsSCS solves/s at 3N=12288 ({head} form; other forms alongside)"
+    for form, f in forms.items():
+        rec = {"solves_per_s": args.steps / f["dt"], "s_per_solve": f["dt"] / args.steps,
+               "e_mbd_ha": f["r"].e_mbd, "iters": f["r"].diag["refine_iters"],
+               "residual": f["r"].diag["residual"],
+               "fallback_used": f["r"].diag["fallback_used"],
+               "kernels_ms": {k: round(v["ms"], 3) for k, v in f["prof"].items() if v["launches"]}}
+        if fp64 is not None:
+            rec["speedup_vs_fp64"] = fp64["s_per_solve"] / rec["s_per_solve"]
+            rec["error_vs_fp64"] = errors(f["r"], r64)
+        out[form] = rec
     if fp64 is not None:
         out["fp64"] = fp64
-        out["speedup_vs_fp64"] = fp64["s_per_solve"] / out["s_per_solve"]
-        out["error_vs_fp64"] = {
-            "e_mbd_ha": abs(rm.e_mbd - r64.e_mbd),
-            "c6_rel_max": float(np.max(np.abs(rm.c6_eff / r64.c6_eff - 1.0))),
-            "alpha_scs_rel_max": float(np.max(np.abs(rm.coupled_pol / r64.coupled_pol - 1.0))),
-            "c6_mol_rel": abs(rm.c6_mol / r64.c6_mol - 1.0)}
-    # kernel breakdown of the profiled mixed solve
-    out["kernels_ms"] = {k: round(v["ms"], 3) for k, v in prof_m.items() if v["launches"]}
+        out["speedup_vs_fp64"] = out[head]["speedup_vs_fp64"]
+        out["error_vs_fp64"] = out[head]["error_vs_fp64"]
     dom = max(prof_m, key=lambda k: prof_m[k]["ms"])
     bound = {"trd_gemv": "hbm", "scs_build": "hbm", "cfdm_build": "hbm", "trsv": "hbm",
-             "refine_residual": "hbm", "chol_syrk_gemm": "fp32_tflops",
+             "refine_residual": "hbm", "pcg_symv": "hbm", "chol_syrk_gemm": "fp32_tflops",
              "chol_trsm_gemm": "fp32_tflops", "trd_syr2k_gemm": "fp64_tflops"}.get(dom, "hbm")
     out["roofline"] = roofline_entry(prof_m, dom, bound)
-    if "trd_gemv" in prof_m and dom != "trd_gemv":
-        out["roofline_trd_gemv"] = roofline_entry(prof_m, "trd_gemv", "hbm")
+    others = {}
+    for k in ("trd_gemv", "pcg_symv", "chol_syrk_gemm"):
+        if k in prof_m and prof_m[k]["launches"] and k != dom:
+            others[k] = roofline_entry(prof_m, k, {"chol_syrk_gemm": "fp32_tflops"}.get(k, "hbm"))
+    out["roofline_other"] = others
     if not args.no_cpu:
         log("bench: CPU baseline sample")
         out["cpu_baseline"] = cpu_baseline(geo, st)

@@ -20,6 +20,7 @@
 #include "../../include/mbd_hip.h"
 #include "common.h"
 #include "kernels.h"
+#include "pcg.h"
 #include "profile.h"
 #include "refine.h"
 
@@ -165,8 +166,11 @@ struct mbd_handle {
   DevBuf<float> A32, L32, inv32;
   DevBuf<int> info;
   DevBuf<double> rnorm;
+  // device: iterative (learned) form
+  DevBuf<double> pMinv, pR, pZ, pP, pQ, pscal, ppart, alpha_pred;
+  bool have_pred = false;
   // device: MBD
-  DevBuf<double> C, om_at, reff, lam, e2, bounds, E, VW, tau, dd, ee, alph, pa, pb;
+  DevBuf<double> C, om_at, reff, lam, e2, bounds, E, VW, tau, dd, ee, alph, pa, pb, py, qred;
   DevBuf<double> ext_alpha, ext_c6;
   DevBuf<int> nneg;
   // instrumentation
@@ -202,9 +206,54 @@ size_t mem_budget() {
   return (size_t)(gb * 1e9);
 }
 
+// Iteration cap of the block-PCG (MBD_PCG_MAXITER overrides; a tiny cap
+// forces the residual gate to fail, which exercises the Cholesky fallback).
+int pcg_maxit() {
+  const char *s = getenv("MBD_PCG_MAXITER");
+  const int v = s ? atoi(s) : 0;
+  return v > 0 ? v : kPcgMaxIter;
+}
+
 int freq_batch(const mbd_handle *h, size_t bytes_per_matrix) {
   size_t b = mem_budget() / std::max<size_t>(bytes_per_matrix, 1);
   return (int)std::max<size_t>(1, std::min<size_t>(kNumFreq, b));
+}
+
+struct DirectStats {
+  int iters = 0;
+  double residual = 0.0;
+};
+
+// FP32 Cholesky of matrices [b0, b0 + bs) of h->A32 (copied to h->L32) and
+// the solve with FP64 iterative refinement against the FP32 tensor; X holds
+// those matrices' solutions (stride sX).
+DirectStats solve_mixed_direct(mbd_handle *h, int n, int lda, long sA, long sinv, long sX,
+                               int b0, int bs, double *X, int *info) {
+  hipStream_t s = h->stream;
+  const float *A = h->A32.p + (long)b0 * sA;
+  float *L = h->L32.p + (long)b0 * sA;
+  float *inv = h->inv32.p + (long)b0 * sinv;
+  double *R = h->R.p + (long)b0 * sX;
+  MBD_HIP(hipMemcpyAsync(L, A, sizeof(float) * (size_t)bs * sA, hipMemcpyDeviceToDevice, s));
+  potrf_batched<float>(s, n, L, lda, sA, bs, inv, sinv, info);
+  set_rhs(s, n, X, n, sX, bs);
+  potrs3_batched<float>(s, n, L, lda, sA, bs, inv, sinv, X, n, sX);
+  DirectStats st;
+  std::vector<double> rn(2 * bs);
+  for (st.iters = 0; st.iters < kRefineMaxIter; ++st.iters) {
+    residual3<float>(s, n, A, lda, sA, bs, X, n, sX, R, n, sX, h->rnorm.p);
+    MBD_HIP(hipMemcpyAsync(rn.data(), h->rnorm.p, sizeof(double) * 2 * bs,
+                           hipMemcpyDeviceToHost, s));
+    MBD_HIP(hipStreamSynchronize(s));
+    // ||b||_inf = 1 for the stacked identity, so max |r| is relative
+    double worst = 0.0;
+    for (int b = 0; b < bs; ++b) worst = std::max(worst, rn[2 * b]);
+    st.residual = worst;
+    if (worst <= kRefineTol) break;
+    potrs3_batched<float>(s, n, L, lda, sA, bs, inv, sinv, R, n, sX);
+    axpy3(s, n, bs, R, n, sX, X, n, sX);
+  }
+  return st;
 }
 
 // SCS at all 21 points into h->alpha (21 x N), h->P (21 x 9), then C6.
@@ -252,39 +301,56 @@ void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
     h->inv32.ensure((size_t)nb * sinv);
     h->R.ensure((size_t)nb * sX);
     h->rnorm.ensure((size_t)2 * kNumFreq);
+    if (mode == MBD_MODE_LEARNED) {
+      const size_t npart = (size_t)std::max(ceil_div(N, 256), ceil_div(n, 16));
+      h->pMinv.ensure((size_t)nb * N * 9);
+      h->pR.ensure((size_t)nb * sX);
+      h->pZ.ensure((size_t)nb * sX);
+      h->pP.ensure((size_t)nb * sX);
+      h->pQ.ensure((size_t)nb * sX);
+      h->pscal.ensure((size_t)nb * kPcgScal);
+      h->ppart.ensure((size_t)nb * npart * 4);
+    }
+    int fallbacks = 0;
     for (int f0 = 0; f0 < kNumFreq; f0 += nb) {
       const int bs = std::min(nb, kNumFreq - f0);
       scs_build<float>(s, N, h->xyz.p, h->nimg_scs, center, h->img_scs.p, rc, h->beta,
                        h->alpha_w.p + (long)f0 * N, h->sgm.p + (long)f0 * N, h->rvdw.p,
                        h->A32.p, lda, sA, bs);
-      MBD_HIP(hipMemcpyAsync(h->L32.p, h->A32.p, sizeof(float) * (size_t)bs * sA,
-                             hipMemcpyDeviceToDevice, s));
-      potrf_batched<float>(s, n, h->L32.p, lda, sA, bs, h->inv32.p, sinv, h->info.p + f0);
       double *Xb = h->X.p + f0 * sX;
-      set_rhs(s, n, Xb, n, sX, bs);
-      potrs3_batched<float>(s, n, h->L32.p, lda, sA, bs, h->inv32.p, sinv, Xb, n, sX);
-      // FP64 iterative refinement against the stored FP32 tensor
-      int it = 0;
-      double last_res = 0.0;
-      std::vector<double> rn(2 * bs);
-      for (; it < kRefineMaxIter; ++it) {
-        residual3<float>(s, n, h->A32.p, lda, sA, bs, Xb, n, sX, h->R.p, n, sX,
-                         h->rnorm.p);
-        MBD_HIP(hipMemcpyAsync(rn.data(), h->rnorm.p, sizeof(double) * 2 * bs,
-                               hipMemcpyDeviceToHost, s));
-        MBD_HIP(hipStreamSynchronize(s));
-        // ||b||_inf = 1 for the stacked identity, so max |r| is relative
-        double worst = 0.0;
-        for (int b = 0; b < bs; ++b) worst = std::max(worst, rn[2 * b]);
-        last_res = worst;
-        if (worst <= kRefineTol) break;
-        potrs3_batched<float>(s, n, h->L32.p, lda, sA, bs, h->inv32.p, sinv, h->R.p, n, sX);
-        axpy3(s, n, bs, h->R.p, n, sX, Xb, n, sX);
+      if (mode == MBD_MODE_MIXED) {
+        DirectStats st = solve_mixed_direct(h, n, lda, sA, sinv, sX, 0, bs, Xb, h->info.p + f0);
+        max_it = std::max(max_it, st.iters);
+        max_res = std::max(max_res, st.residual);
+      } else {
+        PcgWork wk;
+        wk.Minv = h->pMinv.p;
+        wk.R = h->pR.p;
+        wk.Z = h->pZ.p;
+        wk.P = h->pP.p;
+        wk.Q = h->pQ.p;
+        wk.scal = h->pscal.p;
+        wk.part = h->ppart.p;
+        wk.rnorm = h->rnorm.p;
+        const double *a0 = h->have_pred ? h->alpha_pred.p + (long)f0 * N : nullptr;
+        PcgResult pr = pcg_solve<float>(s, N, h->A32.p, lda, sA, bs, a0, Xb, wk, kRefineTol,
+                                        pcg_maxit());
+        for (int b = 0; b < bs; ++b) {
+          max_it = std::max(max_it, pr.iters[b]);
+          if (pr.residual[b] <= kPcgGate) {
+            max_res = std::max(max_res, pr.residual[b]);
+            continue;
+          }
+          // the iterative residual failed: direct FP32 Cholesky + FP64 refinement
+          ++fallbacks;
+          DirectStats st =
+              solve_mixed_direct(h, n, lda, sA, sinv, sX, b, 1, Xb + b * sX, h->info.p + f0 + b);
+          max_res = std::max(max_res, st.residual);
+        }
       }
-      max_it = std::max(max_it, it);
-      max_res = std::max(max_res, last_res);
       contract(s, N, Xb, n, sX, bs, h->alpha.p + (long)f0 * N, h->P.p + 9 * f0);
     }
+    if (d) d->fallback_used = fallbacks;
   }
   h->eig3.ensure(3 * kNumFreq);
   h->c6mol.ensure(1);
@@ -324,8 +390,14 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
   h->dd.ensure(nm);
   h->ee.ensure(nm);
   h->alph.ensure(kTrdNB);
-  h->pa.ensure((size_t)(nm / 256 + 2) * (2 * kTrdNB + 1));
-  h->pb.ensure((size_t)(nm / 16 + 2));
+  wk.pald = (int)(nm / 64 + 2);
+  h->pa.ensure((size_t)wk.pald * (2 * kTrdNB + 1));
+  h->pb.ensure((size_t)wk.pald);
+  const size_t nbt = (size_t)ceil_div(nm, kTrdTile);
+  h->py.ensure(nbt * nbt * kTrdTile);
+  wk.py = h->py.p;
+  h->qred.ensure(2 * kTrdNB + 1);
+  wk.qred = h->qred.p;
   wk.VW = h->VW.p;
   wk.ldv = nm;
   wk.tau = h->tau.p;
@@ -369,7 +441,9 @@ void fetch_scs_outputs(mbd_handle *h, double *alpha_eff, double *coupled_pol,
   MBD_HIP(hipStreamSynchronize(h->stream));
 }
 
-bool valid_mode(int mode) { return mode == MBD_MODE_FP64 || mode == MBD_MODE_MIXED; }
+bool valid_mode(int mode) {
+  return mode == MBD_MODE_FP64 || mode == MBD_MODE_MIXED || mode == MBD_MODE_LEARNED;
+}
 
 }  // namespace
 
@@ -408,7 +482,7 @@ int mbd_destroy(mbd_handle *h) {
                              &h->P, &h->eig3, &h->c6mol, &h->c6, &h->R, &h->D, &h->rnorm,
                              &h->C, &h->om_at, &h->reff, &h->lam, &h->e2, &h->bounds,
                              &h->E, &h->VW, &h->tau, &h->dd, &h->ee, &h->alph, &h->pa,
-                             &h->pb, &h->ext_alpha, &h->ext_c6};
+                             &h->pb, &h->py, &h->qred, &h->pMinv, &h->pR, &h->pZ, &h->pP, &h->pQ, &h->pscal, &h->ppart, &h->alpha_pred, &h->ext_alpha, &h->ext_c6};
   for (auto *b : dbufs) b->release();
   h->A32.release();
   h->L32.release();
@@ -793,8 +867,12 @@ int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w) {
     wk.d = alloc(n);
     wk.e = alloc(n);
     wk.alpha = alloc(kTrdNB);
-    wk.pa = alloc((size_t)(n / 256 + 2) * (2 * kTrdNB + 1));
-    wk.pb = alloc((size_t)(n / 16 + 2));
+    wk.pald = n / 64 + 2;
+    wk.pa = alloc((size_t)wk.pald * (2 * kTrdNB + 1));
+    wk.pb = alloc((size_t)wk.pald);
+    const size_t nbt = (size_t)ceil_div(n, kTrdTile);
+    wk.py = alloc(nbt * nbt * kTrdTile);
+    wk.qred = alloc(2 * kTrdNB + 1);
     tridiagonalize(nullptr, n, dA, n, wk);
     double *e2 = alloc(n), *bounds = alloc(4), *lam = alloc(n);
     tridiag_eigvals(nullptr, n, wk.d, wk.e, e2, bounds, lam);

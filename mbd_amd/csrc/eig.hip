@@ -1,18 +1,22 @@
 // Eigenvalues of the coupled-QHO matrix and E_MBD, all in FP64 (the work
 // PDSYEV('N','L') and the energy loop do at UTILS.f90:835-858).
 //
-// Tridiagonalisation: blocked Householder on the lower triangle with the
-// matrix kept full (both triangles) so that the symmetric product A22 v is a
-// stream of contiguous column dot products.  Per column two kernels:
+// Tridiagonalisation: blocked Householder on the lower triangle (LAPACK's
+// sytrd/latrd organisation), reading and updating only the lower triangle.
+// Per column j three kernels:
 //   colupd: apply the panel's earlier reflectors to column j, finalise the
 //           previous W column's scalar, and emit partial sums of |a|^2,
-//           W^T a and V^T a;
-//   gemv:   form the reflector, y = A22 v (the HBM-bound step), and the W
-//           column; emit partial dots for the next scalar.
+//           W^T a and V^T a (wave-parallel reductions);
+//   symv:   y = A22 v over 128 x 128 lower tiles, the HBM-bound step; each
+//           tile emits its row-dot and column-dot contributions into a
+//           per-tile partial buffer (deterministic, no atomics);
+//   wcol:   reflector scalars, y from the partials, the W and V columns, and
+//           partial dots for the next scalar.
 // W columns are kept un-finalised (W_final = W + alpha_p V) so no kernel
 // rewrites data another workgroup of the same launch reads.  After each panel
-// of NBT columns one MFMA GEMM with K = 2 NBT applies A22 -= [V W][W V]^T.
-// The last <= kTrdTail rows are reduced by one workgroup in LDS.
+// of NBT columns one lower-triangle MFMA SYR2K (GEMM with K = 2 NBT on the
+// lower tiles) applies A22 -= [V W][W V]^T.  The last <= kTrdTail rows are
+// reduced by one workgroup in LDS.
 // Eigenvalues: Sturm-count bisection, one thread per eigenvalue index.
 #include "common.h"
 #include "kernels.h"
@@ -23,33 +27,69 @@ namespace mbd {
 namespace {
 constexpr int NBT = kTrdNB;
 constexpr int PA_STRIDE = 2 * NBT + 1;
-constexpr int GEMV_COLS = 16;  // columns per workgroup (4 per wave)
-constexpr int GEMV_CHUNK = 2048;
+constexpr int TS = kTrdTile;  // symv tile edge
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+
+__device__ inline void tri_index(int idx, int &ti, int &tj) {
+  int t = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+  while ((t + 1) * (t + 2) / 2 <= idx) ++t;
+  while (t * (t + 1) / 2 > idx) --t;
+  ti = t;
+  tj = idx - t * (t + 1) / 2;
+}
+
+// Sum of the npa workgroup partials of quantity q (layout pa[q * pald + w]),
+// reduced by one wave; every lane returns the total.
+__device__ inline double wave_reduce_partials(const double *pa, int pald, int npa, int q,
+                                              int lane) {
+  double acc = 0.0;
+  for (int w = lane; w < npa; w += 64) acc += pa[(long)q * pald + w];
+  return wave_sum(acc);
+}
+
+// Reflector scalars from |a|^2 and the head element: beta, tau and the scale
+// of the stored column (LAPACK dlarfg without its rescaling loop).
+__device__ inline void reflector(double xn2, double alp, double &tau, double &beta,
+                                 double &sc) {
+  if (xn2 == 0.0) {
+    tau = 0.0;
+    beta = alp;
+    sc = 0.0;
+  } else {
+    beta = -copysign(sqrt(alp * alp + xn2), alp);
+    tau = (beta - alp) / beta;
+    sc = 1.0 / (alp - beta);
+  }
+}
 }  // namespace
 
+// 64 rows per workgroup, 4 threads per row (the p-sums split by p mod 4).
 __global__ __launch_bounds__(256) void k_trd_colupd(
     int n, double *__restrict__ A, long lda, double *__restrict__ VW, long ldv,
     int j0, int j, const double *__restrict__ tau, double *__restrict__ alpha_pan,
-    const double *__restrict__ pb_part, int npb, double *__restrict__ pa_part,
+    const double *__restrict__ pb_part, int npb, double *__restrict__ pa, int pald,
     double *__restrict__ d) {
   const int i = j - j0;
   __shared__ double al[NBT], vj[NBT], wj[NBT];
-  __shared__ double sv[256];
-  const int t = threadIdx.x;
+  __shared__ double red[4][64];
+  __shared__ double sv[64];
+  const int t = threadIdx.x, lane = t & 63, part = t >> 6;
   const double *V = VW;
   const double *W = VW + (long)NBT * ldv;
-  if (t == 0 && i > 0) {
-    double s = 0.0;
-    for (int w = 0; w < npb; ++w) s += pb_part[w];
-    const double a = -0.5 * tau[j - 1] * s;
-    al[i - 1] = a;
-    if (blockIdx.x == 0) alpha_pan[i - 1] = a;
+  if (i > 0 && part == 0) {
+    double acc = 0.0;
+    for (int w = lane; w < npb; w += 64) acc += pb_part[w];
+    acc = wave_sum(acc);
+    if (lane == 0) {
+      const double a = -0.5 * tau[j - 1] * acc;
+      al[i - 1] = a;
+      if (blockIdx.x == 0) alpha_pan[i - 1] = a;
+    }
   }
   for (int p = t; p < i - 1; p += blockDim.x) al[p] = alpha_pan[p];
   __syncthreads();
@@ -59,70 +99,187 @@ __global__ __launch_bounds__(256) void k_trd_colupd(
     wj[p] = W[j + (long)p * ldv] + al[p] * v;
   }
   __syncthreads();
-  const int rbase = j + blockIdx.x * blockDim.x;
-  const int r = rbase + t;
+  const int r = j + blockIdx.x * 64 + lane;
   double s = 0.0;
   if (r < n) {
-    s = A[r + (long)j * lda];
-    for (int p = 0; p < i; ++p) {
+#pragma unroll 4
+    for (int p = part; p < i; p += 4) {
       const double vr = V[r + (long)p * ldv];
       const double wr = W[r + (long)p * ldv] + al[p] * vr;
-      s -= vr * wj[p] + wr * vj[p];
+      s += vr * wj[p] + wr * vj[p];
     }
-    A[r + (long)j * lda] = s;
-    if (r == j) d[j] = s;
   }
-  sv[t] = (r >= j + 2 && r < n) ? s : 0.0;
+  red[part][lane] = s;
   __syncthreads();
-  double *out = pa_part + (long)blockIdx.x * PA_STRIDE;
-  for (int q = t; q <= 2 * i; q += blockDim.x) {
-    double acc = 0.0;
-    if (q == 2 * i) {
-      for (int tt = 0; tt < 256; ++tt) acc += sv[tt] * sv[tt];
-    } else {
-      const double *col = (q < i) ? (W + (long)q * ldv) : (V + (long)(q - i) * ldv);
-      const int lim = (n - rbase < 256) ? n - rbase : 256;
-      for (int tt = 0; tt < lim; ++tt) acc += col[rbase + tt] * sv[tt];
+  if (part == 0) {
+    double val = 0.0;
+    if (r < n) {
+      val = A[r + (long)j * lda] - ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
+      A[r + (long)j * lda] = val;
+      if (r == j) d[j] = val;
     }
-    out[q] = acc;
+    sv[lane] = (r >= j + 2 && r < n) ? val : 0.0;
+  }
+  __syncthreads();
+  // partials q < i: W^T a, i <= q < 2i: V^T a, q = 2i: |a|^2; one wave per q
+  const double x = sv[lane];
+  for (int q0 = part * 4; q0 <= 2 * i; q0 += 16) {
+    double acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + u;
+      acc[u] = 0.0;
+      if (q == 2 * i) {
+        acc[u] = x * x;
+      } else if (q < 2 * i && r < n) {
+        const double *col = (q < i) ? (W + (long)q * ldv) : (V + (long)(q - i) * ldv);
+        acc[u] = col[r] * x;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = wave_sum(acc[u]);
+    if (lane == 0) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (q0 + u <= 2 * i) pa[(long)(q0 + u) * pald + blockIdx.x] = acc[u];
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void k_trd_gemv(
-    int n, const double *__restrict__ A, long lda, double *__restrict__ VW,
-    long ldv, int j0, int j, double *__restrict__ tau, double *__restrict__ e,
-    const double *__restrict__ alpha_pan, const double *__restrict__ pa_part,
-    int npa, double *__restrict__ pb_part) {
+// y = A22 v on the lower triangle, tile (I, J), I >= J, of the trailing
+// matrix starting at row/column o = j + 1.  Writes the tile's row dots to
+// Py[I][J] and its column dots to Py[J][I] (on the diagonal, both to Py[I][I]).
+__global__ __launch_bounds__(256) void k_trd_symv(int n, const double *__restrict__ A,
+                                                  long lda, int j0, int j,
+                                                  const double *__restrict__ pa, int pald,
+                                                  int npa, int nbt,
+                                                  double *__restrict__ Py,
+                                                  double *__restrict__ qred) {
+  __shared__ double vI[TS], vJ[TS];
+  __shared__ double rowp[4][TS];
+  __shared__ double colp[TS];
+  __shared__ double sc_s;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int i = j - j0;
+  int I, J;
+  tri_index(blockIdx.x, I, J);
+  const int o = j + 1;
+  const double *colj = A + (long)j * lda;
+  if (blockIdx.x == 0) {
+    // reduce every colupd partial once for k_trd_wcol: qred[0..2i]
+    for (int q0 = wave * 8; q0 <= 2 * i; q0 += 32) {
+      double acc[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+      for (int w = lane; w < npa; w += 64) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (q0 + u <= 2 * i) acc[u] += pa[(long)(q0 + u) * pald + w];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] = wave_sum(acc[u]);
+      if (lane == 0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (q0 + u <= 2 * i) qred[q0 + u] = acc[u];
+      }
+    }
+  }
+  if (wave == 0) {
+    const double xn2 = wave_reduce_partials(pa, pald, npa, 2 * i, lane);
+    if (lane == 0) {
+      double tv, beta, sc;
+      reflector(xn2, colj[o], tv, beta, sc);
+      sc_s = sc;
+    }
+  }
+  __syncthreads();
+  const double sc = sc_s;
+  const int R0 = o + I * TS, C0 = o + J * TS;
+  for (int k = t; k < TS; k += blockDim.x) {
+    const int r = R0 + k, c = C0 + k;
+    vI[k] = (r < n) ? ((r == o) ? 1.0 : sc * colj[r]) : 0.0;
+    vJ[k] = (c < n) ? ((c == o) ? 1.0 : sc * colj[c]) : 0.0;
+  }
+  __syncthreads();
+  const bool diag = (I == J);
+  const int r0 = R0 + lane, r1 = R0 + lane + 64;
+  const double v0 = vI[lane], v1 = vI[lane + 64];
+  double ra0 = 0.0, ra1 = 0.0;
+  constexpr int CPW = TS / 4;  // columns per wave
+  constexpr int G = 8;         // columns whose loads are issued together
+  for (int g = 0; g < CPW; g += G) {
+    double a0[G], a1[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const int c = C0 + wave * CPW + g + u;
+      a0[u] = 0.0;
+      a1[u] = 0.0;
+      if (c < n) {
+        const double *col = A + (long)c * lda;
+        if (r0 < n && (!diag || r0 >= c)) a0[u] = col[r0];
+        if (r1 < n && (!diag || r1 >= c)) a1[u] = col[r1];
+      }
+    }
+    double cs[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const int cl = wave * CPW + g + u;
+      const int c = C0 + cl;
+      const double vc = vJ[cl];
+      ra0 += a0[u] * vc;
+      ra1 += a1[u] * vc;
+      // column dot: strictly-lower elements only on the diagonal tile
+      const double b0 = (diag && r0 == c) ? 0.0 : a0[u];
+      const double b1 = (diag && r1 == c) ? 0.0 : a1[u];
+      cs[u] = b0 * v0 + b1 * v1;
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) cs[u] = wave_sum(cs[u]);
+    if (lane == 0) {
+#pragma unroll
+      for (int u = 0; u < G; ++u) colp[wave * CPW + g + u] = cs[u];
+    }
+  }
+  rowp[wave][lane] = ra0;
+  rowp[wave][lane + 64] = ra1;
+  __syncthreads();
+  double *pIJ = Py + ((long)I * nbt + J) * TS;
+  double *pJI = Py + ((long)J * nbt + I) * TS;
+  for (int k = t; k < TS; k += blockDim.x) {
+    const double rsum = (rowp[0][k] + rowp[1][k]) + (rowp[2][k] + rowp[3][k]);
+    if (diag) {
+      pIJ[k] = rsum + colp[k];
+    } else {
+      pIJ[k] = rsum;
+      pJI[k] = colp[k];
+    }
+  }
+}
+
+// W and V columns for rows o..n-1 from the symv partials; 64 rows per
+// workgroup, 4 threads per row.
+__global__ __launch_bounds__(256) void k_trd_wcol(
+    int n, const double *__restrict__ A, long lda, double *__restrict__ VW, long ldv,
+    int j0, int j, double *__restrict__ tau, double *__restrict__ e,
+    const double *__restrict__ alpha_pan, const double *__restrict__ qred, int nbt,
+    const double *__restrict__ Py, double *__restrict__ pb_part) {
   const int i = j - j0;
   __shared__ double q_s[PA_STRIDE];
   __shared__ double al[NBT], t1[NBT], t2[NBT];
-  __shared__ double vch[GEMV_CHUNK];
-  __shared__ double ycol[GEMV_COLS];
-  __shared__ double pdot[GEMV_COLS];
+  __shared__ double red[4][64];
   __shared__ double sc_s, tau_s;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, part = t >> 6;
   double *V = VW;
   double *W = VW + (long)NBT * ldv;
-  for (int q = t; q <= 2 * i; q += blockDim.x) {
-    double s = 0.0;
-    for (int w = 0; w < npa; ++w) s += pa_part[(long)w * PA_STRIDE + q];
-    q_s[q] = s;
-  }
+  const int o = j + 1;
+  const double *colj = A + (long)j * lda;
+  for (int q = t; q <= 2 * i; q += blockDim.x) q_s[q] = qred[q];
   for (int p = t; p < i; p += blockDim.x) al[p] = alpha_pan[p];
   __syncthreads();
   if (t == 0) {
-    const double xn2 = q_s[2 * i];
-    const double alp = A[(j + 1) + (long)j * lda];
     double tv, beta, sc;
-    if (xn2 == 0.0) {
-      tv = 0.0;
-      beta = alp;
-      sc = 0.0;
-    } else {
-      beta = -copysign(sqrt(alp * alp + xn2), alp);
-      tv = (beta - alp) / beta;
-      sc = 1.0 / (alp - beta);
-    }
+    reflector(q_s[2 * i], colj[o], tv, beta, sc);
     tau_s = tv;
     sc_s = sc;
     if (blockIdx.x == 0) {
@@ -133,65 +290,49 @@ __global__ __launch_bounds__(256) void k_trd_gemv(
   __syncthreads();
   const double sc = sc_s, tv = tau_s;
   for (int p = t; p < i; p += blockDim.x) {
-    const double vh = V[(j + 1) + (long)p * ldv];
-    const double wh = W[(j + 1) + (long)p * ldv] + al[p] * vh;
+    const double vh = V[o + (long)p * ldv];
+    const double wh = W[o + (long)p * ldv] + al[p] * vh;
     const double S2 = q_s[i + p];
     const double S1 = q_s[p] + al[p] * S2;
     t1[p] = wh + sc * S1;
     t2[p] = vh + sc * S2;
   }
-  // y = A22 v for this workgroup's columns
-  const int c0 = j + 1 + blockIdx.x * GEMV_COLS;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const double *colj = A + (long)j * lda;
-  for (int rs = j + 1; rs < n; rs += GEMV_CHUNK) {
-    const int len = (n - rs < GEMV_CHUNK) ? n - rs : GEMV_CHUNK;
-    __syncthreads();
-    for (int k = t; k < len; k += blockDim.x) {
-      const int r = rs + k;
-      vch[k] = (r == j + 1) ? 1.0 : sc * colj[r];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int qc = 0; qc < 4; ++qc) {
-      const int cc = c0 + 4 * wave + qc;
-      if (cc < n) {
-        const double *col = A + (long)cc * lda + rs;
-        double a = 0.0;
-        for (int k = lane; k < len; k += 64) a += col[k] * vch[k];
-        acc[qc] += a;
-      }
-    }
-  }
-#pragma unroll
-  for (int qc = 0; qc < 4; ++qc) {
-    const double s = wave_sum(acc[qc]);
-    if (lane == 0) ycol[4 * wave + qc] = s;
-  }
   __syncthreads();
-  if (t < GEMV_COLS) {
-    const int cc = c0 + t;
+  const int c = o + blockIdx.x * 64 + lane;
+  double part_sum = 0.0;
+  if (c < n) {
+    const int Ib = (c - o) / TS, cl = (c - o) % TS;
+    const double *py = Py + (long)Ib * nbt * TS + cl;
+    double y = 0.0, y2 = 0.0;
+    int Jb = part;
+    for (; Jb + 4 < nbt; Jb += 8) {
+      y += py[(long)Jb * TS];
+      y2 += py[(long)(Jb + 4) * TS];
+    }
+    if (Jb < nbt) y += py[(long)Jb * TS];
+    y += y2;
+    double corr = 0.0;
+#pragma unroll 4
+    for (int p = part; p < i; p += 4) {
+      const double vp = V[c + (long)p * ldv];
+      const double wp = W[c + (long)p * ldv] + al[p] * vp;
+      corr += vp * t1[p] + wp * t2[p];
+    }
+    part_sum = y - corr;
+  }
+  red[part][lane] = part_sum;
+  __syncthreads();
+  if (part == 0) {
     double contrib = 0.0;
-    if (cc < n) {
-      const double vcc = (cc == j + 1) ? 1.0 : sc * colj[cc];
-      double corr = 0.0;
-      for (int p = 0; p < i; ++p) {
-        const double vp = V[cc + (long)p * ldv];
-        const double wp = W[cc + (long)p * ldv] + al[p] * vp;
-        corr += vp * t1[p] + wp * t2[p];
-      }
-      const double wcc = tv * (ycol[t] - corr);
-      W[cc + (long)i * ldv] = wcc;
-      V[cc + (long)i * ldv] = vcc;
+    if (c < n) {
+      const double vcc = (c == o) ? 1.0 : sc * colj[c];
+      const double wcc = tv * ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
+      W[c + (long)i * ldv] = wcc;
+      V[c + (long)i * ldv] = vcc;
       contrib = wcc * vcc;
     }
-    pdot[t] = contrib;
-  }
-  __syncthreads();
-  if (t == 0) {
-    double s = 0.0;
-    for (int q = 0; q < GEMV_COLS; ++q) s += pdot[q];
-    pb_part[blockIdx.x] = s;
+    contrib = wave_sum(contrib);
+    if (lane == 0) pb_part[blockIdx.x] = contrib;
   }
 }
 
@@ -201,13 +342,16 @@ __global__ void k_trd_panel_fin(int n, double *VW, long ldv, int j0, int pb,
                                 const double *tau, double *alpha_pan,
                                 const double *pb_part, int npb) {
   __shared__ double al[NBT];
-  const int t = threadIdx.x;
-  if (t == 0) {
+  const int t = threadIdx.x, lane = t & 63;
+  if (t < 64) {
     double s = 0.0;
-    for (int w = 0; w < npb; ++w) s += pb_part[w];
-    const double a = -0.5 * tau[j0 + pb - 1] * s;
-    al[pb - 1] = a;
-    if (blockIdx.x == 0) alpha_pan[pb - 1] = a;
+    for (int w = lane; w < npb; w += 64) s += pb_part[w];
+    s = wave_sum(s);
+    if (lane == 0) {
+      const double a = -0.5 * tau[j0 + pb - 1] * s;
+      al[pb - 1] = a;
+      if (blockIdx.x == 0) alpha_pan[pb - 1] = a;
+    }
   }
   for (int p = t; p < pb - 1; p += blockDim.x) al[p] = alpha_pan[p];
   __syncthreads();
@@ -221,7 +365,8 @@ __global__ void k_trd_panel_fin(int n, double *VW, long ldv, int j0, int pb,
   }
 }
 
-// Unblocked reduction of the trailing m x m block (m <= kTrdTail) in LDS.
+// Unblocked reduction of the trailing m x m block (m <= kTrdTail) in LDS,
+// loaded from its lower triangle.
 __global__ __launch_bounds__(256) void k_trd_tail(int n, const double *A,
                                                   long lda, int j0, double *d,
                                                   double *e) {
@@ -236,7 +381,8 @@ __global__ __launch_bounds__(256) void k_trd_tail(int n, const double *A,
   const int t = threadIdx.x;
   for (int idx = t; idx < m * m; idx += blockDim.x) {
     const int r = idx % m, c = idx / m;
-    S[c * ld + r] = A[(j0 + r) + (long)(j0 + c) * lda];
+    const int hi = r > c ? r : c, lo = r > c ? c : r;
+    S[c * ld + r] = A[(j0 + hi) + (long)(j0 + lo) * lda];
   }
   __syncthreads();
   for (int k = 0; k < m - 1; ++k) {
@@ -267,7 +413,6 @@ __global__ __launch_bounds__(256) void k_trd_tail(int n, const double *A,
     const double sc = (tv == 0.0) ? 0.0 : 1.0 / (S[k * ld + k + 1] - beta_s);
     for (int r = k + 1 + t; r < m; r += blockDim.x) v[r] = (r == k + 1) ? 1.0 : S[k * ld + r] * sc;
     __syncthreads();
-    // p = tau A22 v
     for (int r = k + 1 + t; r < m; r += blockDim.x) {
       double s = 0.0;
       for (int c = k + 1; c < m; ++c) s += S[c * ld + r] * v[c];
@@ -303,25 +448,32 @@ void tridiagonalize(hipStream_t s, int n, double *A, long lda, TrdWork &wk) {
     int npb = 0;
     for (int i = 0; i < pb; ++i) {
       const int j = j0 + i;
-      const int npa = ceil_div(n - j, 256);
+      const int npa = ceil_div(n - j, 64);
       {
-      ProfScope ps(K_TRD_COLUPD, s, 8.0 * (n - j) * (2.0 * i + 1.0), 4.0 * (n - j) * i);
-      hipLaunchKernelGGL(k_trd_colupd, dim3(npa), dim3(256), 0, s, n, A, lda,
-                         wk.VW, wk.ldv, j0, j, wk.tau, wk.alpha, wk.pb, npb,
-                         wk.pa, wk.d);
-      MBD_LAUNCH_CHECK();
+        ProfScope ps(K_TRD_COLUPD, s, 8.0 * (n - j) * (2.0 * i + 1.0), 4.0 * (n - j) * i);
+        hipLaunchKernelGGL(k_trd_colupd, dim3(npa), dim3(256), 0, s, n, A, lda,
+                           wk.VW, wk.ldv, j0, j, wk.tau, wk.alpha, wk.pb, npb,
+                           wk.pa, wk.pald, wk.d);
+        MBD_LAUNCH_CHECK();
       }
-      const int ngv = ceil_div(n - j - 1, GEMV_COLS);
+      const int m = n - j - 1;
+      const int nbt = ceil_div(m, TS);
       {
-      // algorithmic: the symmetric product reads the lower triangle once
-      const double mj = n - j - 1.0;
-      ProfScope ps(K_TRD_GEMV, s, 8.0 * 0.5 * mj * (mj + 1.0), 2.0 * mj * mj);
-      hipLaunchKernelGGL(k_trd_gemv, dim3(ngv), dim3(256), 0, s, n, A, lda,
-                         wk.VW, wk.ldv, j0, j, wk.tau, wk.e, wk.alpha, wk.pa,
-                         npa, wk.pb);
-      MBD_LAUNCH_CHECK();
+        // algorithmic: the symmetric product reads the lower triangle once
+        ProfScope ps(K_TRD_GEMV, s, 8.0 * 0.5 * m * (m + 1.0), 2.0 * (double)m * m);
+        hipLaunchKernelGGL(k_trd_symv, dim3(nbt * (nbt + 1) / 2), dim3(256), 0, s, n,
+                           A, lda, j0, j, wk.pa, wk.pald, npa, nbt, wk.py, wk.qred);
+        MBD_LAUNCH_CHECK();
       }
-      npb = ngv;
+      const int nwc = ceil_div(m, 64);
+      {
+        ProfScope ps(K_TRD_WCOL, s, 8.0 * m * (2.0 * i + nbt), 4.0 * (double)m * i);
+        hipLaunchKernelGGL(k_trd_wcol, dim3(nwc), dim3(256), 0, s, n, A, lda, wk.VW,
+                           wk.ldv, j0, j, wk.tau, wk.e, wk.alpha, wk.qred, nbt, wk.py,
+                           wk.pb);
+        MBD_LAUNCH_CHECK();
+      }
+      npb = nwc;
     }
     hipLaunchKernelGGL(k_trd_panel_fin, dim3(ceil_div(n - j0 - 1, 256)),
                        dim3(256), 0, s, n, wk.VW, wk.ldv, j0, pb, wk.tau,
@@ -329,7 +481,7 @@ void tridiagonalize(hipStream_t s, int n, double *A, long lda, TrdWork &wk) {
     MBD_LAUNCH_CHECK();
     const int r0 = j0 + pb, m2 = n - r0;
     ProfScope ps(K_TRD_SYR2K, s, 0.0, 2.0 * pb * (double)m2 * (m2 + 1.0));
-    gemm_nt(s, false, m2, m2, 2 * pb, -1.0, wk.VW + r0, (int)wk.ldv, 0,
+    gemm_nt(s, true, m2, m2, 2 * pb, -1.0, wk.VW + r0, (int)wk.ldv, 0,
             wk.VW + (long)NBT * wk.ldv + r0, (int)wk.ldv, 0, 1.0,
             A + r0 + (long)r0 * lda, (int)lda, 0, 1);
     j0 += pb;
@@ -390,33 +542,54 @@ __global__ __launch_bounds__(256) void k_gersh(int n, const double *d,
   }
 }
 
+// Multisection: 16 lanes share one eigenvalue index k and evaluate Sturm
+// counts at 16 interior points of the current interval; the group keeps the
+// sub-interval that holds eigenvalue k (17-fold reduction per sweep) until
+// the interval is within 2 ulp (+ pivmin) of the eigenvalue.
+constexpr int kSect = 16;
+constexpr int kBisectMaxSweeps = 40;
+
+__device__ inline int sturm_count(int n, const double *__restrict__ d,
+                                  const double *__restrict__ e2, double x, double pivmin) {
+  int cnt = 0;
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  cnt += (q < 0.0);
+  for (int i = 1; i < n; ++i) {
+    q = (d[i] - x) - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += (q < 0.0);
+  }
+  return cnt;
+}
+
 __global__ __launch_bounds__(256) void k_bisect(int n, const double *__restrict__ d,
                                                 const double *__restrict__ e2,
                                                 const double *__restrict__ bounds,
                                                 double *__restrict__ w) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = gid / kSect;           // eigenvalue index of this lane group
+  const int sl = gid % kSect;          // lane within the group
+  const int lane = threadIdx.x & 63;
+  const int gshift = (lane / kSect) * kSect;
   const double pivmin = bounds[2];
   double a = bounds[0], b = bounds[1];
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (a + b);
-    if (mid <= a || mid >= b) break;
-    int cnt = 0;
-    double q = d[0] - mid;
-    if (fabs(q) < pivmin) q = -pivmin;
-    cnt += (q < 0.0);
-    for (int i = 1; i < n; ++i) {
-      q = (d[i] - mid) - e2[i - 1] / q;
-      if (fabs(q) < pivmin) q = -pivmin;
-      cnt += (q < 0.0);
-    }
-    if (cnt > k)
-      b = mid;
-    else
-      a = mid;
-    if (b - a <= 2.220446049250313e-16 * fmax(fabs(a), fabs(b)) + pivmin) break;
+  for (int it = 0; it < kBisectMaxSweeps; ++it) {
+    const bool done = (k >= n) ||
+                      (b - a <= 2.220446049250313e-16 * fmax(fabs(a), fabs(b)) + pivmin);
+    if (__all(done)) break;
+    const double h = (b - a) / (double)(kSect + 1);
+    const double x = a + h * (double)(sl + 1);
+    const bool above = (k < n) && sturm_count(n, d, e2, x, pivmin) > k;
+    const unsigned long long bal = __ballot(above);
+    const unsigned bits = (unsigned)((bal >> gshift) & 0xFFFFull);
+    const int sstar = bits ? __builtin_ctz(bits) : kSect;  // first point above k
+    const double na = (sstar == 0) ? a : a + h * (double)sstar;
+    const double nb = (sstar == kSect) ? b : a + h * (double)(sstar + 1);
+    a = na;
+    b = (nb > na) ? nb : na;
   }
-  w[k] = 0.5 * (a + b);
+  if (k < n && sl == 0) w[k] = 0.5 * (a + b);
 }
 
 void tridiag_eigvals(hipStream_t s, int n, const double *d, const double *e,
@@ -424,8 +597,8 @@ void tridiag_eigvals(hipStream_t s, int n, const double *d, const double *e,
   hipLaunchKernelGGL(k_gersh, dim3(1), dim3(256), 0, s, n, d, e, e2, bounds);
   MBD_LAUNCH_CHECK();
   ProfScope ps(K_BISECT, s, 0.0, 0.0);
-  hipLaunchKernelGGL(k_bisect, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, d,
-                     e2, bounds, w);
+  hipLaunchKernelGGL(k_bisect, dim3(ceil_div((long)n * kSect, 256)), dim3(256), 0, s, n,
+                     d, e2, bounds, w);
   MBD_LAUNCH_CHECK();
 }
 

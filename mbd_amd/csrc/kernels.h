@@ -8,6 +8,7 @@ namespace mbd {
 constexpr int kCholNB = 128;   // Cholesky block size (diagonal block in LDS)
 constexpr int kTrdNB = 64;     // Householder panel width
 constexpr int kTrdTail = 128;  // trailing order reduced by one workgroup
+constexpr int kTrdTile = 128;  // tile edge of the lower-triangle symv
 
 // build.hip
 void atom_params(hipStream_t s, int N, int nf, const double *omega,
@@ -49,8 +50,11 @@ struct TrdWork {
   double *d;      // n
   double *e;      // n
   double *alpha;  // kTrdNB
-  double *pa;     // partial sums of colupd, (n/256+1) x (2 kTrdNB + 1)
-  double *pb;     // partial dots of gemv, n/16 + 1
+  double *pa;     // colupd partials, (2 kTrdNB + 1) x pald, q-major
+  int pald;       // >= number of colupd workgroups, n/64 + 2
+  double *pb;     // partial dots of wcol, n/64 + 2
+  double *qred;   // reduced colupd partials, 2 kTrdNB + 1
+  double *py;     // symv tile partials, nbt^2 x kTrdTile, nbt = ceil(n/kTrdTile)
 };
 void tridiagonalize(hipStream_t s, int n, double *A, long lda, TrdWork &wk);
 void tridiag_eigvals(hipStream_t s, int n, const double *d, const double *e,

@@ -21,9 +21,12 @@ enum KernelClass {
   K_CFDM_BUILD,
   K_TRD_COLUPD,
   K_TRD_GEMV,
+  K_TRD_WCOL,
   K_TRD_SYR2K,
   K_TRD_TAIL,
   K_BISECT,
+  K_PCG_SYMV,
+  K_PCG_SMALL,
   K_NUM_CLASSES
 };
 

@@ -7,6 +7,8 @@ namespace mbd {
 
 constexpr int kRefineMaxIter = 8;     // refinement sweeps per frequency batch
 constexpr double kRefineTol = 1e-12;  // max |b - A x| (b = stacked identity)
+constexpr int kPcgMaxIter = 400;      // block-PCG iteration cap per matrix
+constexpr double kPcgGate = 1e-11;    // true residual above which PCG falls back to Cholesky
 
 // R = B - A X for the stacked-identity B, A symmetric full n x n (FP32 or
 // FP64, products accumulated in FP64); rnorm[2 b] = max |R| of matrix b,

@@ -16,9 +16,15 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
+RESULT_LINE = re.compile(r"(\| MBD@rsSCS energy|C6 coefficient|\| ATOM |super cell of dimension|"
+                         r"\| containing|^\s*\|\s+\d+\.\d{6}\s+\S+E)")
+
+
 def parse_out(path):
-    res = {"atoms": {}, "table": []}
+    res = {"atoms": {}, "table": [], "lines": []}
     for line in open(path):
+        if RESULT_LINE.search(line):
+            res["lines"].append(line.rstrip("\n"))
         m = re.search(r"MBD@rsSCS energy\s*:\s*(\S+)\s*Ha\s+(\S+)\s*eV", line)
         if m:
             res["e_mbd_ha"] = float(m.group(1))
