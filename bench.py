@@ -108,6 +108,10 @@ def main():
     log(f"bench: {args.config} N={geo.n_atoms} n={n}")
     eng = MbdEngine(0)
     eng.set_geometry(geo, st)
+    from mbd_amd import learned
+    have_net = os.path.exists(learned.DEFAULT_WEIGHTS)
+    if have_net:
+        eng.net_set_weights(learned.load())
 
     out = {"metric": "MBD@rsSCS solves/s at 3N=12288 (mixed form; FP64 form alongside)",
            "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -145,6 +149,7 @@ def main():
     out["value"] = args.steps / hf["dt"]
     out["ms_per_step"] = 1e3 * hf["dt"] / args.steps
     out["s_per_solve"] = hf["dt"] / args.steps
+    out["network_warm_start"] = have_net
     out["config"] = {"workload": f"{args.config}: {geo.n_atoms}-atom periodic cell, "
                                  f"n_SCS={n}, n_MBD={n * int(np.prod(rm.diag['supercell']))}",
                      "form": head, "scs_cutoff_A": st.mbd_scs_dip_cutoff,

@@ -112,6 +112,37 @@ int mbd_run(mbd_handle *h, int mode, double *e_mbd, double *alpha_eff,
             double *coupled_pol, double *molpol_eig, double *c6_eff,
             double *c6_mol, mbd_diag *d);
 
+/* ---- learned form: per-atom environment network (no counterpart in the
+ * reference; it only warm-starts the iterative SCS solve of MBD_MODE_LEARNED,
+ * whose FP64 residual gate and Cholesky fallback keep the result independent
+ * of the network).  Weights travel as one float "blob": kNetIn feature means,
+ * kNetIn feature standard deviations, then the layer parameters
+ * (W1 b1 W2 b2 W3 b3 W4 b4, each W column-major fo x fi). -------------- */
+
+/* Network layout: inputs per sample, hidden width, floats in a blob. */
+void mbd_net_layout(int *n_in, int *hidden, long *blob_floats);
+
+/* Load weights on the handle (enables the warm start); nullptr unloads. */
+int mbd_net_set_weights(mbd_handle *h, const float *blob, long blob_floats);
+
+/* Descriptors of the current system for all 21 grid points: X[s + d*S],
+ * s = k*n_atoms + i, S = 21*n_atoms, d < n_in; and, if alpha_w is not NULL,
+ * the bare alpha_i(i w_k) in the same sample order. */
+int mbd_net_descriptors(mbd_handle *h, float *X, double *alpha_w);
+
+/* Forward pass y[B] for raw descriptors X[s + d*B] (device `device`). */
+int mbd_net_forward(int device, const float *blob, int B, const float *X, float *y);
+
+/* Loss mean((y - t)^2) and its gradient w.r.t. the layer parameters. */
+int mbd_net_grad(int device, const float *blob, int B, const float *X, const float *t,
+                 float *grad, float *loss);
+
+/* Minibatch Adam on the device: S samples X[s + d*S], targets t[S]; the
+ * blob holds the normalisation and the initial parameters on entry and the
+ * trained parameters on return; loss_hist[epochs] receives the epoch losses. */
+int mbd_net_train(int device, int S, const float *X, const float *t, int epochs, int batch,
+                  float lr, unsigned seed, float *blob, float *loss_hist);
+
 /* ---- instrumentation (no counterpart in the reference; used by bench.py
  * for the per-kernel roofline; the reference only times the whole span with
  * MPI_Wtime, MBD_rsSCS.f90:220-224) ------------------------------------ */

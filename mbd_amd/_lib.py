@@ -67,6 +67,15 @@ def lib():
     L.mbd_scs.argtypes = [h, ctypes.c_int, _D, _D, _D, _D, _D, ctypes.POINTER(MbdDiag)]
     L.mbd_energy.argtypes = [h, _D, _D, _D, _D, _D, ctypes.POINTER(MbdDiag)]
     L.mbd_run.argtypes = [h, ctypes.c_int, _D, _D, _D, _D, _D, _D, ctypes.POINTER(MbdDiag)]
+    _F = ctypes.POINTER(ctypes.c_float)
+    L.mbd_net_layout.argtypes = [_I, _I, ctypes.POINTER(ctypes.c_long)]
+    L.mbd_net_layout.restype = None
+    L.mbd_net_set_weights.argtypes = [h, _F, ctypes.c_long]
+    L.mbd_net_descriptors.argtypes = [h, _F, _D]
+    L.mbd_net_forward.argtypes = [ctypes.c_int, _F, ctypes.c_int, _F, _F]
+    L.mbd_net_grad.argtypes = [ctypes.c_int, _F, ctypes.c_int, _F, _F, _F, _F]
+    L.mbd_net_train.argtypes = [ctypes.c_int, ctypes.c_int, _F, _F, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_float, ctypes.c_uint, _F, _F]
     L.mbd_profile_enable.argtypes = [h, ctypes.c_int]
     L.mbd_profile_read.argtypes = [h, ctypes.c_int, _D, ctypes.POINTER(ctypes.c_char_p)]
     L.mbd_debug_relay_matrix.argtypes = [h, ctypes.c_int, ctypes.c_int, _D]
@@ -79,6 +88,10 @@ def lib():
         getattr(L, name)  # every declared entry point must resolve
     _lib = L
     return L
+
+
+def fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def dptr(a):

@@ -20,6 +20,7 @@
 #include "../../include/mbd_hip.h"
 #include "common.h"
 #include "kernels.h"
+#include "net.h"
 #include "pcg.h"
 #include "profile.h"
 #include "refine.h"
@@ -168,7 +169,12 @@ struct mbd_handle {
   DevBuf<double> rnorm;
   // device: iterative (learned) form
   DevBuf<double> pMinv, pR, pZ, pP, pQ, pscal, ppart, alpha_pred;
-  bool have_pred = false;
+  bool have_net = false;
+  DevBuf<float> net_blob, net_X, net_Xn, net_act;
+  DevBuf<float4> net_nbl;
+  DevBuf<int> net_cnt;
+  DevBuf<double> img_net;  // SCS images that can reach within the descriptor cutoff
+  int nimg_net = 1, center_net = 0;
   // device: MBD
   DevBuf<double> C, om_at, reff, lam, e2, bounds, E, VW, tau, dd, ee, alph, pa, pb, py, qred;
   DevBuf<double> ext_alpha, ext_c6;
@@ -311,6 +317,26 @@ void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
       h->pscal.ensure((size_t)nb * kPcgScal);
       h->ppart.ensure((size_t)nb * npart * 4);
     }
+    if (mode == MBD_MODE_LEARNED && h->have_net) {
+      // network warm start: descriptors and forward pass for every atom and
+      // grid point, alpha_pred = alpha(i w) exp(y)
+      const long S = (long)kNumFreq * N;
+      ProfScope ps(K_NET, s, 0.0, 2.0 * S * (double)net_param_count());
+      h->net_X.ensure((size_t)kNetIn * S);
+      h->net_Xn.ensure((size_t)kNetIn * S);
+      h->net_act.ensure((size_t)S * (3 * kNetHidden + 1));
+      h->alpha_pred.ensure((size_t)S);
+      h->net_nbl.ensure((size_t)N * kNetCap);
+      h->net_cnt.ensure((size_t)N);
+      net_descriptors(s, N, kNumFreq, h->xyz.p, h->nimg_net, h->center_net, h->img_net.p,
+                      h->beta, h->alpha_w.p, h->hirsh.p, h->rvdw.p, h->omega.p, h->net_nbl.p,
+                      h->net_cnt.p, h->net_X.p);
+      mlp_gather(s, (int)S, nullptr, h->net_X.p, S, h->net_blob.p, h->net_Xn.p);
+      float *act[4] = {h->net_act.p, h->net_act.p + S * kNetHidden,
+                       h->net_act.p + 2 * S * kNetHidden, h->net_act.p + 3 * S * kNetHidden};
+      mlp_forward(s, (int)S, h->net_blob.p + 2 * kNetIn, h->net_Xn.p, act);
+      net_alpha(s, (int)S, act[3], h->alpha_w.p, h->alpha_pred.p);
+    }
     int fallbacks = 0;
     for (int f0 = 0; f0 < kNumFreq; f0 += nb) {
       const int bs = std::min(nb, kNumFreq - f0);
@@ -332,7 +358,7 @@ void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
         wk.scal = h->pscal.p;
         wk.part = h->ppart.p;
         wk.rnorm = h->rnorm.p;
-        const double *a0 = h->have_pred ? h->alpha_pred.p + (long)f0 * N : nullptr;
+        const double *a0 = h->have_net ? h->alpha_pred.p + (long)f0 * N : nullptr;
         PcgResult pr = pcg_solve<float>(s, N, h->A32.p, lda, sA, bs, a0, Xb, wk, kRefineTol,
                                         pcg_maxit());
         for (int b = 0; b < bs; ++b) {
@@ -441,6 +467,21 @@ void fetch_scs_outputs(mbd_handle *h, double *alpha_eff, double *coupled_pol,
   MBD_HIP(hipStreamSynchronize(h->stream));
 }
 
+struct DevScratch {
+  std::vector<void *> ptrs;
+  template <typename T>
+  T *get(size_t count) {
+    void *p = nullptr;
+    MBD_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)));
+    ptrs.push_back(p);
+    return static_cast<T *>(p);
+  }
+  ~DevScratch() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+};
+long blob_floats() { return 2L * kNetIn + net_param_count(); }
+
 bool valid_mode(int mode) {
   return mode == MBD_MODE_FP64 || mode == MBD_MODE_MIXED || mode == MBD_MODE_LEARNED;
 }
@@ -482,10 +523,16 @@ int mbd_destroy(mbd_handle *h) {
                              &h->P, &h->eig3, &h->c6mol, &h->c6, &h->R, &h->D, &h->rnorm,
                              &h->C, &h->om_at, &h->reff, &h->lam, &h->e2, &h->bounds,
                              &h->E, &h->VW, &h->tau, &h->dd, &h->ee, &h->alph, &h->pa,
-                             &h->pb, &h->py, &h->qred, &h->pMinv, &h->pR, &h->pZ, &h->pP, &h->pQ, &h->pscal, &h->ppart, &h->alpha_pred, &h->ext_alpha, &h->ext_c6};
+                             &h->pb, &h->py, &h->qred, &h->pMinv, &h->pR, &h->pZ, &h->pP, &h->pQ, &h->pscal, &h->ppart, &h->alpha_pred, &h->img_net, &h->ext_alpha, &h->ext_c6};
   for (auto *b : dbufs) b->release();
   h->A32.release();
   h->L32.release();
+  h->net_blob.release();
+  h->net_X.release();
+  h->net_Xn.release();
+  h->net_act.release();
+  h->net_nbl.release();
+  h->net_cnt.release();
   h->inv32.release();
   h->info.release();
   h->cell.release();
@@ -594,6 +641,28 @@ int mbd_set_system(mbd_handle *h, int n_atoms, const double *xyz, const double *
     up(h->c60, c6_0, n_atoms);
     up(h->r0, r0, n_atoms);
     up(h->img_scs, ti.data(), ti.size());
+    {
+      // descriptor images: r = |x_i - x_j + L| >= |L| - span, with span the
+      // diagonal of the atoms' bounding box, so |L| > cutoff + span never reaches
+      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+      for (int i = 0; i < n_atoms; ++i)
+        for (int c = 0; c < 3; ++c) {
+          lo[c] = std::min(lo[c], xyz[3 * i + c]);
+          hi[c] = std::max(hi[c], xyz[3 * i + c]);
+        }
+      const double span[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+      const double diam = norm3(span);
+      std::vector<double> tn;
+      h->center_net = -1;
+      for (size_t q = 0; q < ti.size() / 3; ++q) {
+        const double L = norm3(&ti[3 * q]);
+        if ((int)q == h->nimg_scs / 2) h->center_net = (int)(tn.size() / 3);
+        if (L <= kNetCut + diam || (int)q == h->nimg_scs / 2)
+          for (int c = 0; c < 3; ++c) tn.push_back(ti[3 * q + c]);
+      }
+      h->nimg_net = (int)(tn.size() / 3);
+      up(h->img_net, tn.data(), tn.size());
+    }
     up(h->img_cfdm, tc.data(), tc.size());
     up(h->lat_d, h->lat, 9);
     h->cell.ensure(cells.size());
@@ -734,6 +803,206 @@ int mbd_profile_read(mbd_handle *h, int cls, double *out, const char **name) {
     out[3] = h->prof.flops[cls];
   }
   if (name) *name = kernel_class_name(cls);
+  return 0;
+}
+
+// ---- learned form: network ------------------------------------------------
+
+
+void mbd_net_layout(int *n_in, int *hidden, long *blob) {
+  if (n_in) *n_in = kNetIn;
+  if (hidden) *hidden = kNetHidden;
+  if (blob) *blob = blob_floats();
+}
+
+int mbd_net_set_weights(mbd_handle *h, const float *blob, long n) {
+  if (!h) return -1;
+  if (!blob) {
+    h->have_net = false;
+    return 0;
+  }
+  if (n != blob_floats()) return -3;
+  try {
+    MBD_HIP(hipSetDevice(h->device));
+    h->net_blob.ensure((size_t)n);
+    MBD_HIP(hipMemcpy(h->net_blob.p, blob, sizeof(float) * n, hipMemcpyHostToDevice));
+    h->have_net = true;
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_net_descriptors(mbd_handle *h, float *X, double *alpha_w) {
+  if (!h) return -1;
+  if (!X) return -2;
+  if (!h->has_sys) return MBD_ERR_NOSYS;
+  try {
+    MBD_HIP(hipSetDevice(h->device));
+    const int N = h->N;
+    const long S = (long)kNumFreq * N;
+    hipStream_t s = h->stream;
+    h->alpha_w.ensure((size_t)S);
+    h->sgm.ensure((size_t)S);
+    h->rvdw.ensure(N);
+    atom_params(s, N, kNumFreq, h->omega.p, h->hirsh.p, h->a0.p, h->c60.p, h->r0.p,
+                h->alpha_w.p, h->sgm.p, h->rvdw.p);
+    h->net_X.ensure((size_t)kNetIn * S);
+    h->net_nbl.ensure((size_t)N * kNetCap);
+    h->net_cnt.ensure((size_t)N);
+    net_descriptors(s, N, kNumFreq, h->xyz.p, h->nimg_net, h->center_net, h->img_net.p, h->beta,
+                    h->alpha_w.p, h->hirsh.p, h->rvdw.p, h->omega.p, h->net_nbl.p, h->net_cnt.p,
+                    h->net_X.p);
+    MBD_HIP(hipMemcpyAsync(X, h->net_X.p, sizeof(float) * kNetIn * S, hipMemcpyDeviceToHost, s));
+    if (alpha_w)
+      MBD_HIP(hipMemcpyAsync(alpha_w, h->alpha_w.p, sizeof(double) * S, hipMemcpyDeviceToHost, s));
+    MBD_HIP(hipStreamSynchronize(s));
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_net_forward(int device, const float *blob, int B, const float *X, float *y) {
+  if (!blob) return -2;
+  if (B <= 0) return -3;
+  if (!X) return -4;
+  if (!y) return -5;
+  try {
+    MBD_HIP(hipSetDevice(device));
+    DevScratch sc;
+    float *db = sc.get<float>(blob_floats());
+    float *dX = sc.get<float>((size_t)kNetIn * B);
+    float *dXn = sc.get<float>((size_t)kNetIn * B);
+    float *act0 = sc.get<float>((size_t)B * (3 * kNetHidden + 1));
+    MBD_HIP(hipMemcpy(db, blob, sizeof(float) * blob_floats(), hipMemcpyHostToDevice));
+    MBD_HIP(hipMemcpy(dX, X, sizeof(float) * kNetIn * B, hipMemcpyHostToDevice));
+    float *act[4] = {act0, act0 + (long)B * kNetHidden, act0 + 2L * B * kNetHidden,
+                     act0 + 3L * B * kNetHidden};
+    mlp_gather(nullptr, B, nullptr, dX, B, db, dXn);
+    mlp_forward(nullptr, B, db + 2 * kNetIn, dXn, act);
+    MBD_HIP(hipDeviceSynchronize());
+    MBD_HIP(hipMemcpy(y, act[3], sizeof(float) * B, hipMemcpyDeviceToHost));
+  } catch (const HipError &e) {
+    return fail(nullptr, e);
+  }
+  return 0;
+}
+
+int mbd_net_grad(int device, const float *blob, int B, const float *X, const float *t,
+                 float *grad, float *loss) {
+  if (!blob) return -2;
+  if (B <= 0) return -3;
+  if (!X || !t) return -4;
+  if (!grad) return -6;
+  try {
+    MBD_HIP(hipSetDevice(device));
+    DevScratch sc;
+    const long np = net_param_count();
+    float *db = sc.get<float>(blob_floats());
+    float *dX = sc.get<float>((size_t)kNetIn * B);
+    float *dXn = sc.get<float>((size_t)kNetIn * B);
+    float *dt = sc.get<float>(B);
+    float *g = sc.get<float>(B);
+    float *G = sc.get<float>(np);
+    float *lp = sc.get<float>(ceil_div(B, 256));
+    float *act0 = sc.get<float>((size_t)B * (3 * kNetHidden + 1));
+    float *del0 = sc.get<float>((size_t)B * (3 * kNetHidden + 1));
+    MBD_HIP(hipMemcpy(db, blob, sizeof(float) * blob_floats(), hipMemcpyHostToDevice));
+    MBD_HIP(hipMemcpy(dX, X, sizeof(float) * kNetIn * B, hipMemcpyHostToDevice));
+    MBD_HIP(hipMemcpy(dt, t, sizeof(float) * B, hipMemcpyHostToDevice));
+    float *act[4] = {act0, act0 + (long)B * kNetHidden, act0 + 2L * B * kNetHidden,
+                     act0 + 3L * B * kNetHidden};
+    float *del[4] = {del0, del0 + (long)B * kNetHidden, del0 + 2L * B * kNetHidden,
+                     del0 + 3L * B * kNetHidden};
+    mlp_gather(nullptr, B, nullptr, dX, B, db, dXn);
+    mlp_forward(nullptr, B, db + 2 * kNetIn, dXn, act);
+    mlp_loss(nullptr, B, act[3], nullptr, dt, g, lp);
+    mlp_backward(nullptr, B, db + 2 * kNetIn, dXn, act, del, g, G);
+    MBD_HIP(hipDeviceSynchronize());
+    MBD_HIP(hipMemcpy(grad, G, sizeof(float) * np, hipMemcpyDeviceToHost));
+    if (loss) {
+      std::vector<float> l(ceil_div(B, 256));
+      MBD_HIP(hipMemcpy(l.data(), lp, sizeof(float) * l.size(), hipMemcpyDeviceToHost));
+      double acc = 0.0;
+      for (float v : l) acc += v;
+      *loss = (float)(acc / B);
+    }
+  } catch (const HipError &e) {
+    return fail(nullptr, e);
+  }
+  return 0;
+}
+
+int mbd_net_train(int device, int S, const float *X, const float *t, int epochs, int batch,
+                  float lr, unsigned seed, float *blob, float *loss_hist) {
+  if (S <= 0) return -2;
+  if (!X) return -3;
+  if (!t) return -4;
+  if (epochs <= 0) return -5;
+  if (batch <= 0) return -6;
+  if (!blob) return -9;
+  try {
+    MBD_HIP(hipSetDevice(device));
+    DevScratch sc;
+    const long np = net_param_count();
+    const int B = std::min(batch, S);
+    float *db = sc.get<float>(blob_floats());
+    float *dX = sc.get<float>((size_t)kNetIn * S);
+    float *dt = sc.get<float>(S);
+    int *didx = sc.get<int>(S);
+    float *dXn = sc.get<float>((size_t)kNetIn * B);
+    float *g = sc.get<float>(B);
+    float *G = sc.get<float>(np);
+    float *m = sc.get<float>(np);
+    float *v = sc.get<float>(np);
+    const int nlp = ceil_div(B, 256);
+    const int nbatch = ceil_div(S, B);
+    float *lp = sc.get<float>((size_t)nlp * nbatch);
+    float *act0 = sc.get<float>((size_t)B * (3 * kNetHidden + 1));
+    float *del0 = sc.get<float>((size_t)B * (3 * kNetHidden + 1));
+    MBD_HIP(hipMemcpy(db, blob, sizeof(float) * blob_floats(), hipMemcpyHostToDevice));
+    MBD_HIP(hipMemcpy(dX, X, sizeof(float) * kNetIn * S, hipMemcpyHostToDevice));
+    MBD_HIP(hipMemcpy(dt, t, sizeof(float) * S, hipMemcpyHostToDevice));
+    MBD_HIP(hipMemset(m, 0, sizeof(float) * np));
+    MBD_HIP(hipMemset(v, 0, sizeof(float) * np));
+    float *act[4] = {act0, act0 + (long)B * kNetHidden, act0 + 2L * B * kNetHidden,
+                     act0 + 3L * B * kNetHidden};
+    float *del[4] = {del0, del0 + (long)B * kNetHidden, del0 + 2L * B * kNetHidden,
+                     del0 + 3L * B * kNetHidden};
+    std::vector<int> perm(S);
+    for (int q = 0; q < S; ++q) perm[q] = q;
+    unsigned long long st = seed ? seed : 1;
+    int step = 0;
+    std::vector<float> lh((size_t)nlp * nbatch);
+    for (int ep = 0; ep < epochs; ++ep) {
+      for (int q = S - 1; q > 0; --q) {  // Fisher-Yates with xorshift64
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        std::swap(perm[q], perm[st % (unsigned long long)(q + 1)]);
+      }
+      MBD_HIP(hipMemcpy(didx, perm.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+      MBD_HIP(hipMemset(lp, 0, sizeof(float) * lh.size()));
+      for (int bi = 0; bi < nbatch; ++bi) {
+        const int b0 = bi * B, bb = std::min(B, S - b0);
+        mlp_gather(nullptr, bb, didx + b0, dX, S, db, dXn);
+        mlp_forward(nullptr, bb, db + 2 * kNetIn, dXn, act);
+        mlp_loss(nullptr, bb, act[3], didx + b0, dt, g, lp + (long)bi * nlp);
+        mlp_backward(nullptr, bb, db + 2 * kNetIn, dXn, act, del, g, G);
+        adam_step(nullptr, (int)np, db + 2 * kNetIn, G, m, v, lr, ++step);
+      }
+      MBD_HIP(hipDeviceSynchronize());
+      MBD_HIP(hipMemcpy(lh.data(), lp, sizeof(float) * lh.size(), hipMemcpyDeviceToHost));
+      double acc = 0.0;
+      for (float x : lh) acc += x;
+      if (loss_hist) loss_hist[ep] = (float)(acc / S);
+    }
+    MBD_HIP(hipMemcpy(blob + 2 * kNetIn, db + 2 * kNetIn, sizeof(float) * np,
+                      hipMemcpyDeviceToHost));
+  } catch (const HipError &e) {
+    return fail(nullptr, e);
+  }
   return 0;
 }
 

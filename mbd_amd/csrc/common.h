@@ -55,4 +55,9 @@ void gemm_nt(hipStream_t s, bool lower, int m, int n, int k, float alpha,
              const float *A, int lda, long sA, const float *B, int ldb,
              long sB, float beta, float *C, int ldc, long sC, int batch);
 
+// FP32 C = alpha op(A) op(B)^T + beta C (single matrix) where op(A) is m x k
+// and op(B) is n x k: ta reads A as k x m, tb reads B as k x n.
+void gemm_f32(hipStream_t s, int ta, int tb, int m, int n, int k, float alpha, const float *A,
+              int lda, const float *B, int ldb, float beta, float *C, int ldc);
+
 }  // namespace mbd

@@ -43,11 +43,13 @@ struct Mfma<float> {
 constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
 constexpr int PER_T = BM * BK / NT;  // elements of each operand per thread
 
+// ta: op(A)[r][k] = A[k + r lda] instead of A[r + k lda];
+// tb: op(B)[c][k] = B[k + c ldb] instead of B[c + k ldb].
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(NT) void k_gemm_nt(
     int m, int n, int k, T alpha, const T *__restrict__ A, int lda, long sA,
     const T *__restrict__ B, int ldb, long sB, T beta, T *__restrict__ C,
-    int ldc, long sC, int tiles_n) {
+    int ldc, long sC, int tiles_n, int ta, int tb) {
   int tm, tn;
   const int idx = blockIdx.x;
   if (LOWER) {
@@ -86,8 +88,8 @@ __global__ __launch_bounds__(NT) void k_gemm_nt(
       const int r = e & (BM - 1), kk = e / BM;
       const int gk = k0 + kk;
       const int gr = row0 + r, gc = col0 + r;
-      ra[i] = (gr < m && gk < k) ? A[gr + (long)gk * lda] : T(0);
-      rb[i] = (gc < n && gk < k) ? B[gc + (long)gk * ldb] : T(0);
+      ra[i] = (gr < m && gk < k) ? (ta ? A[gk + (long)gr * lda] : A[gr + (long)gk * lda]) : T(0);
+      rb[i] = (gc < n && gk < k) ? (tb ? B[gk + (long)gc * ldb] : B[gc + (long)gk * ldb]) : T(0);
     }
   };
   auto sstore = [&](int buf) {
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(NT) void k_gemm_nt(
 template <typename T>
 void launch(hipStream_t s, bool lower, int m, int n, int k, T alpha,
             const T *A, int lda, long sA, const T *B, int ldb, long sB,
-            T beta, T *C, int ldc, long sC, int batch) {
+            T beta, T *C, int ldc, long sC, int batch, int ta = 0, int tb = 0) {
   if (m <= 0 || n <= 0 || batch <= 0) return;
   const int tmn = ceil_div(m, BM), tnn = ceil_div(n, BN);
   if (lower) {
@@ -151,16 +153,21 @@ void launch(hipStream_t s, bool lower, int m, int n, int k, T alpha,
     const int tiles = tmn * (tmn + 1) / 2;
     dim3 grid(tiles, 1, batch);
     hipLaunchKernelGGL((k_gemm_nt<T, true>), grid, dim3(NT), 0, s, m, n, k,
-                       alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, tnn);
+                       alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, tnn, ta, tb);
   } else {
     dim3 grid(tmn * tnn, 1, batch);
     hipLaunchKernelGGL((k_gemm_nt<T, false>), grid, dim3(NT), 0, s, m, n, k,
-                       alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, tnn);
+                       alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, tnn, ta, tb);
   }
   MBD_LAUNCH_CHECK();
 }
 
 }  // namespace
+
+void gemm_f32(hipStream_t s, int ta, int tb, int m, int n, int k, float alpha, const float *A,
+              int lda, const float *B, int ldb, float beta, float *C, int ldc) {
+  launch<float>(s, false, m, n, k, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, ta, tb);
+}
 
 void gemm_nt(hipStream_t s, bool lower, int m, int n, int k, double alpha,
              const double *A, int lda, long sA, const double *B, int ldb,

@@ -27,6 +27,7 @@ enum KernelClass {
   K_BISECT,
   K_PCG_SYMV,
   K_PCG_SMALL,
+  K_NET,
   K_NUM_CLASSES
 };
 

@@ -17,7 +17,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import MbdDiag, dptr, iptr
+from ._lib import MbdDiag, dptr, fptr, iptr
 
 MODES = {"fp64": _lib.MODE_FP64, "mixed": _lib.MODE_MIXED, "learned": _lib.MODE_LEARNED}
 
@@ -185,6 +185,25 @@ class MbdEngine:
         self._check(rc, "mbd_energy")
         return e.value, om, rf, _diag_dict(d)
 
+    # -- learned form -----------------------------------------------------------
+    def net_set_weights(self, blob):
+        """Load network weights (float32 blob, see net_layout); None unloads."""
+        if blob is None:
+            self._check(self._L.mbd_net_set_weights(self._h, None, 0), "mbd_net_set_weights")
+            return
+        b = np.ascontiguousarray(blob, dtype=np.float32)
+        self._check(self._L.mbd_net_set_weights(self._h, fptr(b), b.size), "mbd_net_set_weights")
+
+    def net_descriptors(self):
+        """(X, alpha_w): descriptors (21 N, n_in) and bare alpha_i(i w_k) (21 N),
+        sample s = k N + i."""
+        n_in, _, _ = net_layout()
+        S = 21 * self.n_atoms
+        X = np.zeros(n_in * S, dtype=np.float32)
+        aw = np.zeros(S)
+        self._check(self._L.mbd_net_descriptors(self._h, fptr(X), dptr(aw)), "mbd_net_descriptors")
+        return X.reshape(n_in, S).T.copy(), aw
+
     # -- instrumentation -------------------------------------------------------
     def profile_enable(self, on: bool = True):
         self._check(self._L.mbd_profile_enable(self._h, int(on)), "mbd_profile_enable")
@@ -257,3 +276,54 @@ def device_gemm_nt(A, B, C, alpha=1.0, beta_=0.0, fp32=False, device: int = 0):
     if rc != 0:
         raise MbdError(f"mbd_debug_gemm_nt failed ({rc})")
     return Cf
+
+
+def net_layout():
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+    _lib.lib().mbd_net_layout(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return a.value, b.value, c.value
+
+
+def _feat_major(X):
+    """(B, n_in) samples -> the ABI's sample-fastest layout X[s + d B]."""
+    return np.ascontiguousarray(np.asarray(X, dtype=np.float32).T)
+
+
+def net_forward(blob, X, device: int = 0):
+    b = np.ascontiguousarray(blob, dtype=np.float32)
+    Xf = _feat_major(X)
+    B = Xf.shape[1]
+    y = np.zeros(B, dtype=np.float32)
+    rc = _lib.lib().mbd_net_forward(device, fptr(b), B, fptr(Xf), fptr(y))
+    if rc != 0:
+        raise MbdError(f"mbd_net_forward failed ({rc})")
+    return y
+
+
+def net_grad(blob, X, t, device: int = 0):
+    b = np.ascontiguousarray(blob, dtype=np.float32)
+    Xf = _feat_major(X)
+    B = Xf.shape[1]
+    tt = np.ascontiguousarray(t, dtype=np.float32)
+    _, _, nb = net_layout()
+    n_in = Xf.shape[0]
+    g = np.zeros(nb - 2 * n_in, dtype=np.float32)
+    loss = ctypes.c_float()
+    rc = _lib.lib().mbd_net_grad(device, fptr(b), B, fptr(Xf), fptr(tt), fptr(g), ctypes.byref(loss))
+    if rc != 0:
+        raise MbdError(f"mbd_net_grad failed ({rc})")
+    return g, loss.value
+
+
+def net_train(blob, X, t, epochs=30, batch=1024, lr=1e-3, seed=1, device: int = 0):
+    """Train on the device; returns (trained blob, per-epoch losses)."""
+    b = np.array(blob, dtype=np.float32, copy=True)
+    Xf = _feat_major(X)
+    S = Xf.shape[1]
+    tt = np.ascontiguousarray(t, dtype=np.float32)
+    hist = np.zeros(epochs, dtype=np.float32)
+    rc = _lib.lib().mbd_net_train(device, S, fptr(Xf), fptr(tt), epochs, batch, lr, seed,
+                                  fptr(b), fptr(hist))
+    if rc != 0:
+        raise MbdError(f"mbd_net_train failed ({rc})")
+    return b, hist
