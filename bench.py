@@ -66,6 +66,39 @@ def roofline_entry(prof, cls, bound):
             / max(k["launches"], 1)}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trd_symv_c3_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or
+    None when no summary covers it."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        t = line.strip().strip("`").split()
+        if len(t) == 3 and t[0] == f"traffic_bytes_per_launch[{kernel}]" and t[1] == "=":
+            return float(t[2])
+    return None
+
+
+def traffic_probe(args):
+    """Workload of the hardware-counter runs: one 1 GiB copy with the plain
+    copy kernel (the known-size reference), then one solve of the headline
+    form.  Meant to run under rocprofv3 --pmc; prints progress only."""
+    from mbd_amd import engine, generators, learned
+    ms = engine.device_copy(1 << 30)
+    log(f"traffic-probe: copy 1 GiB in {ms:.3f} ms")
+    geo, st = generators.CONFIGS[args.config]()
+    eng = engine.MbdEngine(0)
+    eng.set_geometry(geo, st)
+    if os.path.exists(learned.DEFAULT_WEIGHTS):
+        eng.net_set_weights(learned.load())
+    r = eng.run(args.headline)
+    log(f"traffic-probe: {args.headline} E={r.e_mbd:.10f}")
+    eng.close()
+
+
 def cpu_baseline(geo, st, sample_eig_n=3072):
     """Bounded sample of the oracle on the host cores: one SCS frequency at the
     full size (tensor build + Cholesky + 3-RHS solve), and the eigenvalue stage
@@ -94,7 +127,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--headline", default="learned", choices=["mixed", "learned"])
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="workload for rocprofv3 --pmc runs (no JSON line)")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding "
                                    "(by frequency) is a design note only", "n_gpus": args.gpus}))
@@ -171,12 +209,13 @@ def main():
         out["speedup_vs_fp64"] = out[head]["speedup_vs_fp64"]
         out["error_vs_fp64"] = out[head]["error_vs_fp64"]
     dom = max(prof_m, key=lambda k: prof_m[k]["ms"])
-    bound = {"trd_gemv": "hbm", "scs_build": "hbm", "cfdm_build": "hbm", "trsv": "hbm",
+    bound = {"trd_symv": "hbm", "scs_build": "hbm", "cfdm_build": "hbm", "trsv": "hbm",
              "refine_residual": "hbm", "pcg_symv": "hbm", "chol_syrk_gemm": "fp32_tflops",
              "chol_trsm_gemm": "fp32_tflops", "trd_syr2k_gemm": "fp64_tflops"}.get(dom, "hbm")
     out["roofline"] = roofline_entry(prof_m, dom, bound)
+    out["roofline"]["traffic"] = measured_traffic(dom)
     others = {}
-    for k in ("trd_gemv", "pcg_symv", "chol_syrk_gemm"):
+    for k in ("trd_symv", "pcg_symv", "chol_syrk_gemm"):
         if k in prof_m and prof_m[k]["launches"] and k != dom:
             others[k] = roofline_entry(prof_m, k, {"chol_syrk_gemm": "fp32_tflops"}.get(k, "hbm"))
     out["roofline_other"] = others

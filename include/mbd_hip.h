@@ -41,6 +41,7 @@ typedef struct mbd_handle mbd_handle;
 #define MBD_ERR_NOSYS 1002   /* mbd_set_system not called */
 #define MBD_ERR_NOMEM 1003   /* device allocation failed */
 #define MBD_ERR_NONET 1004   /* MBD_MODE_LEARNED without network weights */
+#define MBD_ERR_TOOBIG 1005  /* molecule larger than the batched kernel holds */
 
 /* Diagnostics of one call (all times in milliseconds of device time). */
 typedef struct {
@@ -184,6 +185,56 @@ int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w);
 int mbd_debug_gemm_nt(int device, int fp32, int m, int n, int k,
                       double alpha, const double *A, const double *B,
                       double beta, double *C);
+
+/* ---- Batched molecules ----------------------------------------------------
+ * Many independent non-periodic molecules solved together: every
+ * (molecule, grid point) SCS problem and every molecule's coupled-QHO
+ * eigenproblem run in ONE persistent kernel launch, replayed from a captured
+ * hipGraph.  Per molecule the results are what mbd_run returns for that
+ * molecule alone (the reference runs one system per pmbd.x invocation:
+ * MBD_rsSCS.f90:220-222 around MBD_at_rsSCS, UTILS.f90:153-349).
+ * Modes: MBD_MODE_FP64, or MBD_MODE_MIXED (FP32 Cholesky, FP64 refinement
+ * against the FP64 matrix, FP64 fallback per grid point; the eigen stage is
+ * FP64 in both). */
+typedef struct {
+  int config;          /* candidate index used (mbd_batch_config) */
+  int nb, nt, grid;    /* FP32 panel width, threads per workgroup, workgroups */
+  int fallbacks;       /* grid points that fell back to the FP64 Cholesky */
+  int max_iters;       /* largest number of refinement sweeps */
+  int n_failed;        /* molecules with a non-positive FP64 pivot */
+  int n_negative;      /* negative coupled-QHO eigenvalues, all molecules */
+  double max_residual; /* largest final max|B - A X| */
+  double t_ms;         /* device time of the replayed graph */
+} mbd_batch_diag;
+
+/* offsets[nmol + 1] (offsets[0] = 0) split the atom arrays into molecules;
+ * xyz[3 * i + c] in bohr; alpha0, c6_0, r0 from mbd_free_atom_params;
+ * cutoffs in Angstrom as in settings.in.  Returns MBD_ERR_TOOBIG (see
+ * mbd_last_error) if a molecule exceeds the kernel's LDS capacity. */
+int mbd_batch_set(mbd_handle *h, int nmol, const int *offsets, const double *xyz,
+                  const double *hirshfeld, const double *alpha0, const double *c6_0,
+                  const double *r0, double beta, double rc_scs, double rc_cfdm);
+
+/* Autotuner: times every candidate launch shape that fits the current batch
+ * (reps launches each) and keeps the fastest for later mbd_batch_run calls.
+ * times_ms[kBatchNumConfigs = 8] receives the average per candidate (-1 if
+ * not eligible); *best the chosen index. */
+int mbd_batch_tune(mbd_handle *h, int mode, int reps, double *times_ms, int *best);
+
+/* Candidate table of the autotuner. */
+int mbd_batch_config(int index, int *nb, int *nt, int *per_cu);
+
+/* Runs the batch.  energy[nmol] (Ha); optional: alpha_eff[natoms],
+ * c6_eff[natoms], c6_mol[nmol], alpha_all[21 * natoms] (alpha_SCS at every
+ * grid point, [k * natoms + i]), stat[4 * nmol] (FP64 pivot info, negative
+ * eigenvalues, refinement sweeps, fallbacks), d. */
+int mbd_batch_run(mbd_handle *h, int mode, double *energy, double *alpha_eff, double *c6_eff,
+                  double *c6_mol, double *alpha_all, int *stat, mbd_batch_diag *d);
+
+/* Copies n_bytes (a multiple of 16) between two fresh device buffers with a
+ * plain streaming kernel (k_copy_probe) and returns its duration in *ms:
+ * the known-size reference that hardware byte counters are scaled by. */
+int mbd_debug_copy(int device, long n_bytes, double *ms);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,13 @@ class MbdDiag(ctypes.Structure):
                 ("t_mbd_ms", ctypes.c_double), ("t_total_ms", ctypes.c_double)]
 
 
+class MbdBatchDiag(ctypes.Structure):
+    _fields_ = [("config", ctypes.c_int), ("nb", ctypes.c_int), ("nt", ctypes.c_int),
+                ("grid", ctypes.c_int), ("fallbacks", ctypes.c_int), ("max_iters", ctypes.c_int),
+                ("n_failed", ctypes.c_int), ("n_negative", ctypes.c_int),
+                ("max_residual", ctypes.c_double), ("t_ms", ctypes.c_double)]
+
+
 class LibraryMissing(RuntimeError):
     pass
 
@@ -84,6 +91,13 @@ def lib():
     L.mbd_debug_sym_eigvals.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D]
     L.mbd_debug_gemm_nt.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_double, _D, _D, ctypes.c_double, _D]
+    L.mbd_batch_set.argtypes = [h, ctypes.c_int, _I, _D, _D, _D, _D, _D, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_double]
+    L.mbd_batch_tune.argtypes = [h, ctypes.c_int, ctypes.c_int, _D, _I]
+    L.mbd_batch_config.argtypes = [ctypes.c_int, _I, _I, _I]
+    L.mbd_batch_run.argtypes = [h, ctypes.c_int, _D, _D, _D, _D, _D, _I,
+                                ctypes.POINTER(MbdBatchDiag)]
+    L.mbd_debug_copy.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.POINTER(ctypes.c_double)]
     for name in declared_symbols():
         getattr(L, name)  # every declared entry point must resolve
     _lib = L

@@ -15,6 +15,7 @@
 #include "common.h"
 #include "kernels.h"
 #include "profile.h"
+#include "tensor.h"
 
 namespace mbd {
 
@@ -30,24 +31,6 @@ __device__ inline void tri_index(int idx, int &ti, int &tj) {
   while (t * (t + 1) / 2 > idx) --t;
   ti = t;
   tj = idx - t * (t + 1) / 2;
-}
-
-// r = sqrt(dx^2 + dy^2 + dz^2) with two-operand rounding, as the reference's
-// DSQRT(dxyz(1)*dxyz(1) + dxyz(2)*dxyz(2) + dxyz(3)*dxyz(3)).
-__device__ inline double exact_dist(double dx, double dy, double dz) {
-#pragma clang fp contract(off)
-  double s = dx * dx + dy * dy;
-  s = s + dz * dz;
-  return sqrt(s);
-}
-
-__device__ inline void add3(double a0, double a1, double a2, double t0,
-                            double t1, double t2, double &r0, double &r1,
-                            double &r2) {
-#pragma clang fp contract(off)
-  r0 = a0 + t0;
-  r1 = a1 + t1;
-  r2 = a2 + t2;
 }
 
 // Six unique entries of a symmetric 3x3: xx, yy, zz, xy, xz, yz.
@@ -73,27 +56,6 @@ struct Comp6 {
   __device__ float get(int i) const { return s[i] + c[i]; }
 };
 
-// SR tensor (UTILS.f90:663-692) in FP64, accumulated into acc.
-__device__ inline void sr_tensor_add64(double dx, double dy, double dz,
-                                       double r, double sgm, double rvdw,
-                                       double beta, double *acc) {
-  const double fermi = r / (beta * rvdw);
-  const double x = r / sgm;
-  const double ex = exp(-(x * x));
-  const double r2 = r * r;
-  const double zl = (erf(x) - sqrt(4.0 / kPi) * x * ex) / (r2 * r2 * r);
-  const double zr = (sqrt(16.0 / kPi) * ex) / (sgm * sgm * sgm * r2);
-  const double damp = 1.0 - (1.0 / (1.0 + exp(-6.0 * (fermi - 1.0))));
-  const double d[3] = {dx, dy, dz};
-  const int rr[6] = {0, 1, 2, 0, 0, 1}, cc[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-  for (int e = 0; e < 6; ++e) {
-    const double dd = d[rr[e]] * d[cc[e]];
-    const double t = 3.0 * dd - (e < 3 ? r2 : 0.0);
-    acc[e] += (-t * zl + dd * zr) * damp;
-  }
-}
-
 // SR tensor in FP32 with the cancellation-free damping e/(1+e).
 __device__ inline void sr_tensor_add32(float dx, float dy, float dz, float r,
                                        float sgm, float rvdw, float beta,
@@ -113,23 +75,6 @@ __device__ inline void sr_tensor_add32(float dx, float dy, float dz, float r,
     const float dd = d[rr[e]] * d[cc[e]];
     const float t = 3.0f * dd - (e < 3 ? r2 : 0.0f);
     acc.add(e, (-t * zl + dd * zr) * damp);
-  }
-}
-
-// LR tensor (UTILS.f90:1140-1150) times pref, accumulated.
-__device__ inline void lr_tensor_add64(double dx, double dy, double dz,
-                                       double r, double rvdw, double beta,
-                                       double pref, double *acc) {
-  const double fermi = r / (beta * rvdw);
-  const double f = 1.0 / (1.0 + exp(-6.0 * (fermi - 1.0)));
-  const double r2 = r * r;
-  const double r5 = r2 * r2 * r;
-  const double d[3] = {dx, dy, dz};
-  const int rr[6] = {0, 1, 2, 0, 0, 1}, cc[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-  for (int e = 0; e < 6; ++e) {
-    const double t = -3.0 * d[rr[e]] * d[cc[e]] + (e < 3 ? r2 : 0.0);
-    acc[e] += ((t / r5) * f) * pref;
   }
 }
 

@@ -21,6 +21,7 @@
 #include "common.h"
 #include "kernels.h"
 #include "net.h"
+#include "batch.h"
 #include "pcg.h"
 #include "profile.h"
 #include "refine.h"
@@ -179,6 +180,17 @@ struct mbd_handle {
   DevBuf<double> C, om_at, reff, lam, e2, bounds, E, VW, tau, dd, ee, alph, pa, pb, py, qred;
   DevBuf<double> ext_alpha, ext_c6;
   DevBuf<int> nneg;
+  // batched molecules (one persistent launch, replayed from a hipGraph)
+  struct Batch {
+    int nmol = 0, natoms = 0, nmax = 0, ncu = 0;
+    double beta = 1.0, rc_scs = 0.0, rc_cfdm = 0.0;  // bohr
+    DevBuf<int> off, order, counter, stat;
+    DevBuf<double> xyz, hirsh, a0, c60, r0, alpha, c6, energy, c6_mol, resid;
+    DevBuf<char> work;
+    int cfg = -1;  // tuned candidate, -1 = default
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int exec_cfg[2] = {-1, -1};
+  } bt;
   // instrumentation
   bool prof_on = false;
   Profiler prof;
@@ -195,6 +207,28 @@ struct ProfGuard {
 }  // namespace
 
 namespace {
+
+void batch_drop_graphs(mbd_handle *h) {
+  for (int m = 0; m < 2; ++m) {
+    if (h->bt.exec[m]) (void)hipGraphExecDestroy(h->bt.exec[m]);
+    h->bt.exec[m] = nullptr;
+    h->bt.exec_cfg[m] = -1;
+  }
+}
+
+void batch_release(mbd_handle *h) {
+  batch_drop_graphs(h);
+  auto &b = h->bt;
+  DevBuf<double> *d[] = {&b.xyz, &b.hirsh, &b.a0, &b.c60, &b.r0, &b.alpha,
+                         &b.c6, &b.energy, &b.c6_mol, &b.resid};
+  for (auto *x : d) x->release();
+  b.off.release();
+  b.order.release();
+  b.counter.release();
+  b.stat.release();
+  b.work.release();
+  b.nmol = 0;
+}
 
 int fail(mbd_handle *h, const HipError &e) {
   char buf[256];
@@ -537,6 +571,7 @@ int mbd_destroy(mbd_handle *h) {
   h->info.release();
   h->cell.release();
   h->nneg.release();
+  batch_release(h);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return 0;
@@ -1194,6 +1229,301 @@ int mbd_debug_gemm_nt(int device, int fp32, int m, int n, int k, double alpha, c
   (void)hipFree(dA);
   (void)hipFree(dB);
   (void)hipFree(dC);
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+__global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ src,
+                                                     float4 *__restrict__ dst, long n) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+    dst[q] = src[q];
+}
+}  // namespace
+
+extern "C" int mbd_debug_copy(int device, long n_bytes, double *ms) {
+  if (n_bytes <= 0 || n_bytes % 16) return -2;
+  void *src = nullptr, *dst = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  try {
+    MBD_HIP(hipSetDevice(device));
+    MBD_HIP(hipMalloc(&src, n_bytes));
+    MBD_HIP(hipMalloc(&dst, n_bytes));
+    MBD_HIP(hipMemset(src, 0, n_bytes));
+    MBD_HIP(hipEventCreate(&e0));
+    MBD_HIP(hipEventCreate(&e1));
+    const long n = n_bytes / 16;
+    MBD_HIP(hipEventRecord(e0, nullptr));
+    hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, nullptr, (const float4 *)src,
+                       (float4 *)dst, n);
+    MBD_LAUNCH_CHECK();
+    MBD_HIP(hipEventRecord(e1, nullptr));
+    MBD_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    MBD_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+  } catch (const HipError &e) {
+    rc = fail(nullptr, e);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (src) (void)hipFree(src);
+  if (dst) (void)hipFree(dst);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Batched molecules.
+namespace {
+
+int batch_default_config(int nmax) {
+  const int pref[] = {4, 0, 6, 2};  // {32,256,1}, {16,256,1}, {32,512,1}, {16,512,1}
+  for (int c : pref)
+    if (batch_lds_bytes(kBatchConfigs[c], nmax)) return c;
+  return -1;
+}
+
+BatchArgs batch_args(mbd_handle *h, int mode) {
+  auto &b = h->bt;
+  BatchArgs a;
+  a.nmol = b.nmol;
+  a.ntask = b.nmol * kNumFreq;
+  a.nmax = b.nmax;
+  a.S = batch_lds_stride(b.nmax);
+  a.natoms = b.natoms;
+  a.mode = mode == MBD_MODE_FP64 ? 0 : 1;
+  a.off = b.off.p;
+  a.order = b.order.p;
+  a.xyz = b.xyz.p;
+  a.hirsh = b.hirsh.p;
+  a.a0 = b.a0.p;
+  a.c60 = b.c60.p;
+  a.r0 = b.r0.p;
+  a.omega = h->omega.p;
+  a.weight = h->weight.p;
+  a.beta = b.beta;
+  a.rc_scs = b.rc_scs;
+  a.rc_cfdm = b.rc_cfdm;
+  a.work = b.work.p;
+  a.slot_bytes = (long)batch_slot_bytes(b.nmax);
+  a.alpha = b.alpha.p;
+  a.c6 = b.c6.p;
+  a.energy = b.energy.p;
+  a.c6_mol = b.c6_mol.p;
+  a.stat = b.stat.p;
+  a.resid = b.resid.p;
+  a.counter = b.counter.p;
+  return a;
+}
+
+// Zero the counters and per-molecule statistics, then the persistent kernel.
+void batch_enqueue(mbd_handle *h, int mode, int cfg) {
+  auto &b = h->bt;
+  const BatchConfig &c = kBatchConfigs[cfg];
+  batch_zero(h->stream, b.counter.p, 1 + b.nmol);
+  batch_zero(h->stream, b.stat.p, 4 * b.nmol);
+  batch_zero(h->stream, (int *)b.resid.p, 2 * b.nmol);
+  batch_launch(h->stream, c, batch_args(h, mode), b.ncu * c.per_cu);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbd_batch_config(int index, int *nb, int *nt, int *per_cu) {
+  if (index < 0 || index >= kBatchNumConfigs) return -1;
+  if (nb) *nb = kBatchConfigs[index].nb;
+  if (nt) *nt = kBatchConfigs[index].nt;
+  if (per_cu) *per_cu = kBatchConfigs[index].per_cu;
+  return 0;
+}
+
+int mbd_batch_set(mbd_handle *h, int nmol, const int *offsets, const double *xyz,
+                  const double *hirshfeld, const double *alpha0, const double *c6_0,
+                  const double *r0, double beta, double rc_scs, double rc_cfdm) {
+  if (!h) return -1;
+  if (nmol <= 0) return -2;
+  if (!offsets) return -3;
+  if (!xyz) return -4;
+  if (!hirshfeld) return -5;
+  if (!alpha0) return -6;
+  if (!c6_0) return -7;
+  if (!r0) return -8;
+  if (offsets[0] != 0) return -3;
+  int nmax = 0;
+  for (int m = 0; m < nmol; ++m) {
+    const int na = offsets[m + 1] - offsets[m];
+    if (na <= 0) return -3;
+    nmax = std::max(nmax, 3 * na);
+  }
+  if (batch_default_config(nmax) < 0) {
+    char buf[160];
+    snprintf(buf, sizeof buf,
+             "batched kernel holds molecules up to 3N = %d in LDS; largest here is 3N = %d",
+             [] {
+               int lo = 3;
+               while (batch_default_config(lo + 3) >= 0) lo += 3;
+               return lo;
+             }(),
+             nmax);
+    h->err = buf;
+    return MBD_ERR_TOOBIG;
+  }
+  try {
+    MBD_HIP(hipSetDevice(h->device));
+    MBD_HIP(hipStreamSynchronize(h->stream));
+    batch_drop_graphs(h);
+    auto &b = h->bt;
+    b.nmol = nmol;
+    b.natoms = offsets[nmol];
+    b.nmax = nmax;
+    b.beta = beta;
+    b.rc_scs = rc_scs / kBohr;
+    b.rc_cfdm = rc_cfdm / kBohr;
+    MBD_HIP(hipDeviceGetAttribute(&b.ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+    std::vector<int> order(nmol);
+    for (int m = 0; m < nmol; ++m) order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      return offsets[x + 1] - offsets[x] > offsets[y + 1] - offsets[y];
+    });
+    const int NA = b.natoms;
+    auto upd = [&](DevBuf<double> &d, const double *src, size_t cnt) {
+      d.ensure(cnt);
+      MBD_HIP(hipMemcpyAsync(d.p, src, sizeof(double) * cnt, hipMemcpyHostToDevice, h->stream));
+    };
+    auto upi = [&](DevBuf<int> &d, const int *src, size_t cnt) {
+      d.ensure(cnt);
+      MBD_HIP(hipMemcpyAsync(d.p, src, sizeof(int) * cnt, hipMemcpyHostToDevice, h->stream));
+    };
+    upi(b.off, offsets, nmol + 1);
+    upi(b.order, order.data(), nmol);
+    upd(b.xyz, xyz, 3L * NA);
+    upd(b.hirsh, hirshfeld, NA);
+    upd(b.a0, alpha0, NA);
+    upd(b.c60, c6_0, NA);
+    upd(b.r0, r0, NA);
+    b.alpha.ensure((size_t)kNumFreq * NA);
+    b.c6.ensure(NA);
+    b.energy.ensure(nmol);
+    b.c6_mol.ensure(nmol);
+    b.resid.ensure(nmol);
+    b.counter.ensure(1 + nmol);
+    b.stat.ensure(4 * nmol);
+    b.work.ensure(2 * (size_t)b.ncu * batch_slot_bytes(nmax));
+    MBD_HIP(hipStreamSynchronize(h->stream));
+    if (b.cfg >= 0 && !batch_lds_bytes(kBatchConfigs[b.cfg], nmax)) b.cfg = -1;
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_batch_tune(mbd_handle *h, int mode, int reps, double *times_ms, int *best) {
+  if (!h) return -1;
+  if (mode != MBD_MODE_FP64 && mode != MBD_MODE_MIXED) return -2;
+  if (reps <= 0) return -3;
+  if (!h->bt.nmol) return MBD_ERR_NOSYS;
+  try {
+    MBD_HIP(hipSetDevice(h->device));
+    int arg = -1;
+    double tbest = 0.0;
+    for (int c = 0; c < kBatchNumConfigs; ++c) {
+      double t = -1.0;
+      if (batch_lds_bytes(kBatchConfigs[c], h->bt.nmax)) {
+        batch_prepare(kBatchConfigs[c], h->bt.nmax);
+        batch_enqueue(h, mode, c);  // warm-up
+        Timer tm;
+        tm.start(h->stream);
+        for (int r = 0; r < reps; ++r) batch_enqueue(h, mode, c);
+        tm.stop(h->stream);
+        t = tm.ms() / reps;
+        if (arg < 0 || t < tbest) {
+          arg = c;
+          tbest = t;
+        }
+      }
+      if (times_ms) times_ms[c] = t;
+    }
+    h->bt.cfg = arg;
+    if (best) *best = arg;
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_batch_run(mbd_handle *h, int mode, double *energy, double *alpha_eff, double *c6_eff,
+                  double *c6_mol, double *alpha_all, int *stat, mbd_batch_diag *d) {
+  if (!h) return -1;
+  if (mode != MBD_MODE_FP64 && mode != MBD_MODE_MIXED) return -2;
+  if (!energy) return -3;
+  auto &b = h->bt;
+  if (!b.nmol) return MBD_ERR_NOSYS;
+  try {
+    MBD_HIP(hipSetDevice(h->device));
+    const int cfg = b.cfg >= 0 ? b.cfg : batch_default_config(b.nmax);
+    const int mi = mode == MBD_MODE_FP64 ? 0 : 1;
+    if (!b.exec[mi] || b.exec_cfg[mi] != cfg) {
+      if (b.exec[mi]) (void)hipGraphExecDestroy(b.exec[mi]);
+      b.exec[mi] = nullptr;
+      batch_prepare(kBatchConfigs[cfg], b.nmax);
+      hipGraph_t g = nullptr;
+      MBD_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+      try {
+        batch_enqueue(h, mode, cfg);
+      } catch (const HipError &) {
+        (void)hipStreamEndCapture(h->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      MBD_HIP(hipStreamEndCapture(h->stream, &g));
+      const hipError_t ie = hipGraphInstantiate(&b.exec[mi], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      MBD_HIP(ie);
+      b.exec_cfg[mi] = cfg;
+    }
+    Timer tm;
+    tm.start(h->stream);
+    MBD_HIP(hipGraphLaunch(b.exec[mi], h->stream));
+    tm.stop(h->stream);
+    const double ms = tm.ms();
+    const int NA = b.natoms, nm = b.nmol;
+    std::vector<int> st(4 * (size_t)nm);
+    std::vector<double> res(nm);
+    auto get = [&](void *dst, const void *src, size_t bytes) {
+      MBD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+    };
+    get(energy, b.energy.p, sizeof(double) * nm);
+    get(st.data(), b.stat.p, sizeof(int) * 4 * nm);
+    get(res.data(), b.resid.p, sizeof(double) * nm);
+    if (alpha_eff) get(alpha_eff, b.alpha.p, sizeof(double) * NA);
+    if (alpha_all) get(alpha_all, b.alpha.p, sizeof(double) * kNumFreq * NA);
+    if (c6_eff) get(c6_eff, b.c6.p, sizeof(double) * NA);
+    if (c6_mol) get(c6_mol, b.c6_mol.p, sizeof(double) * nm);
+    MBD_HIP(hipStreamSynchronize(h->stream));
+    if (stat) std::copy(st.begin(), st.end(), stat);
+    if (d) {
+      const BatchConfig &c = kBatchConfigs[cfg];
+      d->config = cfg;
+      d->nb = c.nb;
+      d->nt = c.nt;
+      d->grid = b.ncu * c.per_cu;
+      d->fallbacks = d->max_iters = d->n_failed = d->n_negative = 0;
+      d->max_residual = 0.0;
+      for (int m = 0; m < nm; ++m) {
+        d->n_failed += st[4 * m] != 0;
+        d->n_negative += st[4 * m + 1];
+        d->max_iters = std::max(d->max_iters, st[4 * m + 2]);
+        d->fallbacks += st[4 * m + 3];
+        d->max_residual = std::max(d->max_residual, res[m]);
+      }
+      d->t_ms = ms;
+    }
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
   return 0;
 }
 

@@ -460,7 +460,7 @@ void tridiagonalize(hipStream_t s, int n, double *A, long lda, TrdWork &wk) {
       const int nbt = ceil_div(m, TS);
       {
         // algorithmic: the symmetric product reads the lower triangle once
-        ProfScope ps(K_TRD_GEMV, s, 8.0 * 0.5 * m * (m + 1.0), 2.0 * (double)m * m);
+        ProfScope ps(K_TRD_SYMV, s, 8.0 * 0.5 * m * (m + 1.0), 2.0 * (double)m * m);
         hipLaunchKernelGGL(k_trd_symv, dim3(nbt * (nbt + 1) / 2), dim3(256), 0, s, n,
                            A, lda, j0, j, wk.pa, wk.pald, npa, nbt, wk.py, wk.qred);
         MBD_LAUNCH_CHECK();

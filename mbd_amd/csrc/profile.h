@@ -20,7 +20,7 @@ enum KernelClass {
   K_REFINE,
   K_CFDM_BUILD,
   K_TRD_COLUPD,
-  K_TRD_GEMV,
+  K_TRD_SYMV,
   K_TRD_WCOL,
   K_TRD_SYR2K,
   K_TRD_TAIL,

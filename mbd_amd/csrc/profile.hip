@@ -8,7 +8,7 @@ namespace mbd {
 const char *kernel_class_name(int cls) {
   static const char *names[K_NUM_CLASSES] = {
       "scs_build", "chol_diag", "chol_trsm_gemm", "chol_syrk_gemm", "trsv",
-      "refine_residual", "cfdm_build", "trd_colupd", "trd_gemv", "trd_wcol", "trd_syr2k_gemm",
+      "refine_residual", "cfdm_build", "trd_colupd", "trd_symv", "trd_wcol", "trd_syr2k_gemm",
       "trd_tail", "bisect", "pcg_symv", "pcg_small", "net_infer"};
   return (cls >= 0 && cls < K_NUM_CLASSES) ? names[cls] : "";
 }

@@ -17,7 +17,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import MbdDiag, dptr, fptr, iptr
+from ._lib import MbdBatchDiag, MbdDiag, dptr, fptr, iptr
 
 MODES = {"fp64": _lib.MODE_FP64, "mixed": _lib.MODE_MIXED, "learned": _lib.MODE_LEARNED}
 
@@ -185,6 +185,68 @@ class MbdEngine:
         self._check(rc, "mbd_energy")
         return e.value, om, rf, _diag_dict(d)
 
+    # -- batched molecules -------------------------------------------------------
+    def batch_set(self, molecules, settings=None):
+        """Many non-periodic molecules (mbd_amd.io.Geometry, Angstrom) for the
+        batched kernel; settings gives xc and the SCS / CFDM cutoffs."""
+        from .io import BOHR, Settings
+        st = settings or Settings()
+        offs = [0]
+        xyz, hir, a0, c6, r0 = [], [], [], [], []
+        for g in molecules:
+            if g.periodic:
+                raise ValueError("the batched form takes non-periodic molecules")
+            for nm in g.names:
+                c, a, r = free_atom_params(nm)
+                c6.append(c)
+                a0.append(a)
+                r0.append(r)
+            xyz.append(np.asarray(g.coords_ang, dtype=np.float64) / BOHR)
+            hir.append(np.asarray(g.hirshfeld, dtype=np.float64))
+            offs.append(offs[-1] + g.n_atoms)
+        self.batch_set_params(np.array(offs, dtype=np.int32), np.concatenate(xyz),
+                              np.concatenate(hir), np.array(a0), np.array(c6), np.array(r0),
+                              beta(st.flag_xc), st.mbd_scs_dip_cutoff, st.mbd_cfdm_dip_cutoff)
+
+    def batch_set_params(self, offsets, xyz_bohr, hirshfeld, alpha0, c6_0, r0, beta_,
+                         rc_scs=120.0, rc_cfdm=100.0):
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        arrs = [np.ascontiguousarray(x, dtype=np.float64)
+                for x in (xyz_bohr, hirshfeld, alpha0, c6_0, r0)]
+        rc = self._L.mbd_batch_set(self._h, len(off) - 1, iptr(off), *[dptr(x) for x in arrs],
+                                   float(beta_), float(rc_scs), float(rc_cfdm))
+        self._check(rc, "mbd_batch_set")
+        self._batch_off = off
+
+    def batch_tune(self, mode: str = "mixed", reps: int = 3):
+        """Times every eligible launch shape; returns (best index, times in ms)."""
+        t = np.zeros(8)
+        best = ctypes.c_int()
+        self._check(self._L.mbd_batch_tune(self._h, MODES[mode], reps, dptr(t), ctypes.byref(best)),
+                    "mbd_batch_tune")
+        return best.value, t
+
+    def batch_run(self, mode: str = "mixed", full: bool = True) -> dict:
+        off = self._batch_off
+        nmol, na = len(off) - 1, int(off[-1])
+        e = np.zeros(nmol)
+        out = {"energy": e}
+        d = MbdBatchDiag()
+        if full:
+            ae, c6, c6m, aa = np.zeros(na), np.zeros(na), np.zeros(nmol), np.zeros(21 * na)
+            st = np.zeros(4 * nmol, dtype=np.int32)
+            rc = self._L.mbd_batch_run(self._h, MODES[mode], dptr(e), dptr(ae), dptr(c6),
+                                       dptr(c6m), dptr(aa), iptr(st), ctypes.byref(d))
+            out.update(alpha_eff=ae, c6_eff=c6, c6_mol=c6m, alpha_all=aa.reshape(21, na),
+                       stat=st.reshape(nmol, 4))
+        else:
+            rc = self._L.mbd_batch_run(self._h, MODES[mode], dptr(e), None, None, None, None, None,
+                                       ctypes.byref(d))
+        self._check(rc, "mbd_batch_run")
+        out["diag"] = {f: getattr(d, f) for f, _ in MbdBatchDiag._fields_}
+        out["offsets"] = off
+        return out
+
     # -- learned form -----------------------------------------------------------
     def net_set_weights(self, blob):
         """Load network weights (float32 blob, see net_layout); None unloads."""
@@ -276,6 +338,23 @@ def device_gemm_nt(A, B, C, alpha=1.0, beta_=0.0, fp32=False, device: int = 0):
     if rc != 0:
         raise MbdError(f"mbd_debug_gemm_nt failed ({rc})")
     return Cf
+
+
+def device_copy(n_bytes: int, device: int = 0) -> float:
+    """Plain device-to-device copy of n_bytes with the library's streaming
+    kernel; returns its duration in ms."""
+    ms = ctypes.c_double()
+    rc = _lib.lib().mbd_debug_copy(device, int(n_bytes), ctypes.byref(ms))
+    if rc != 0:
+        raise MbdError(f"mbd_debug_copy failed ({rc})")
+    return ms.value
+
+
+def batch_config(index: int):
+    nb, nt, pc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    if _lib.lib().mbd_batch_config(index, ctypes.byref(nb), ctypes.byref(nt), ctypes.byref(pc)):
+        raise IndexError(index)
+    return {"nb": nb.value, "nt": nt.value, "per_cu": pc.value}
 
 
 def net_layout():

@@ -181,3 +181,39 @@ CONFIGS = {
     "c3_random_4096": lambda: random_periodic(4096, 38.0, seed=4096),
     "c5_hex_sheet_4096": lambda: hex_sheet(64, 32),
 }
+
+
+def write_config(name: str, outdir: str) -> List[str]:
+    """Write a configuration as pmbd.x inputs under ``outdir``: geometry.in and
+    settings.in, or for the batched form mol_XXX.in (one per molecule) and a
+    shared settings.in.  Returns the written paths."""
+    import os
+
+    from .io import write_geometry, write_settings
+    os.makedirs(outdir, exist_ok=True)
+    paths = []
+    if name == "c4_chains_512":
+        mols, st = chain_molecules()
+        for k, g in enumerate(mols):
+            p = os.path.join(outdir, f"mol_{k:03d}.in")
+            write_geometry(p, g.names, g.coords_ang, g.hirshfeld, g.lattice_ang)
+            paths.append(p)
+    else:
+        g, st = CONFIGS[name]()
+        p = os.path.join(outdir, "geometry.in")
+        write_geometry(p, g.names, g.coords_ang, g.hirshfeld, g.lattice_ang)
+        paths.append(p)
+    p = os.path.join(outdir, "settings.in")
+    write_settings(p, st)
+    paths.append(p)
+    return paths
+
+
+if __name__ == "__main__":
+    import sys
+    if len(sys.argv) != 3 or sys.argv[1] not in list(CONFIGS) + ["c4_chains_512"]:
+        print(f"usage: python -m mbd_amd.generators "
+              f"{{{'|'.join(list(CONFIGS) + ['c4_chains_512'])}}} OUTDIR", file=sys.stderr)
+        sys.exit(2)
+    for path in write_config(sys.argv[1], sys.argv[2]):
+        print(path)
