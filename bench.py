@@ -99,6 +99,54 @@ def traffic_probe(args):
     eng.close()
 
 
+def batched_line(steps, warmup):
+    """Batched form: 512 chain molecules (8-256 atoms) per launch, mixed
+    precision, autotuned launch shape, hipGraph replays; the FP64 batch and a
+    sample of the single-system path alongside."""
+    from mbd_amd import engine, generators
+    mols, st = generators.chain_molecules()
+    eng = engine.MbdEngine(0)
+    eng.batch_set(mols, st)
+    best, times = eng.batch_tune("mixed", reps=2)
+    out = {"workload": "c4_chains_512: 512 non-periodic chains, 8-256 atoms, all 21 points",
+           "molecules": len(mols), "atoms": int(sum(m.n_atoms for m in mols)),
+           "tuned": {"index": best, **engine.batch_config(best)},
+           "tuner_ms": [round(t, 3) for t in times]}
+    res = {}
+    for mode in ("mixed", "fp64"):
+        for _ in range(warmup):
+            eng.batch_run(mode, full=False)
+        t0 = time.perf_counter()
+        dev = 0.0
+        for _ in range(steps):
+            r = eng.batch_run(mode, full=False)
+            dev += r["diag"]["t_ms"]
+        dt = (time.perf_counter() - t0) / steps
+        res[mode] = eng.batch_run(mode)
+        out[mode] = {"ms_per_batch": 1e3 * dt, "device_ms_per_batch": dev / steps,
+                     "molecules_per_s": len(mols) / dt,
+                     "fallbacks": res[mode]["diag"]["fallbacks"],
+                     "max_iters": res[mode]["diag"]["max_iters"]}
+        log(f"bench: batched {mode} {1e3 * dt:.2f} ms per 512 molecules")
+    out["error_vs_fp64"] = {
+        "e_mbd_ha_max": float(np.max(np.abs(res["mixed"]["energy"] - res["fp64"]["energy"]))),
+        "c6_rel_max": float(np.max(np.abs(res["mixed"]["c6_eff"] / res["fp64"]["c6_eff"] - 1)))}
+    eng.close()
+    # single-system path on every 8th molecule, scaled to the batch
+    single = engine.MbdEngine(0)
+    t_single = 0.0
+    for g in mols[::8]:
+        single.set_geometry(g, st)
+        single.run_energy_only("mixed")
+        t0 = time.perf_counter()
+        single.run_energy_only("mixed")
+        t_single += time.perf_counter() - t0
+    single.close()
+    out["single_system_path_s_per_batch_est"] = 8 * t_single
+    out["speedup_vs_single_system_path"] = 8 * t_single / (out["mixed"]["ms_per_batch"] * 1e-3)
+    return out
+
+
 def cpu_baseline(geo, st, sample_eig_n=3072):
     """Bounded sample of the oracle on the host cores: one SCS frequency at the
     full size (tensor build + Cholesky + 3-RHS solve), and the eigenvalue stage
@@ -127,6 +175,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--headline", default="learned", choices=["mixed", "learned"])
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="workload for rocprofv3 --pmc runs (no JSON line)")
     args = ap.parse_args()
@@ -219,10 +268,12 @@ def main():
         if k in prof_m and prof_m[k]["launches"] and k != dom:
             others[k] = roofline_entry(prof_m, k, {"chol_syrk_gemm": "fp32_tflops"}.get(k, "hbm"))
     out["roofline_other"] = others
+    eng.close()
+    if not args.no_batch:
+        out["batched"] = batched_line(args.steps, args.warmup)
     if not args.no_cpu:
         log("bench: CPU baseline sample")
         out["cpu_baseline"] = cpu_baseline(geo, st)
-    eng.close()
     print(json.dumps(out), flush=True)
 
 

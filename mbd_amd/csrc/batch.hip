@@ -168,33 +168,47 @@ __device__ int wg_potrf(T *L, int ld, int n, T *P, int S, T *Z, T *Linv) {
       const int c = idx / m, r = idx - c * m;
       if (r >= c) L[(k0 + r) + (long)(k0 + c) * ld] = P[c * S + r];
     }
-    // trailing lower triangle: L22 -= P2 P2^T in 16 x 16 MFMA tiles
+    // trailing lower triangle: L22 -= P2 P2^T in 16 x 16 MFMA tiles, four
+    // tiles per wave at a time so their loads are in flight together
     const int mt = m - kb;
     if (mt > 0) {
+      constexpr int TQ = 4;
       const int tr = (mt + 15) / 16;
       const int ntile = tr * (tr + 1) / 2;
       const int kk4 = (kb + 3) & ~3;
       const int g = lane >> 4, l16 = lane & 15;
-      for (int t = wave; t < ntile; t += NW) {
-        int bi, bj;
-        tri_rc(t, bi, bj);
-        const int r0 = kb + 16 * bi, c0 = kb + 16 * bj;  // panel-relative
-        typename M::acc_t acc;
-        const int col = k0 + c0 + l16;
+      for (int t0 = TQ * wave; t0 < ntile; t0 += TQ * NW) {
+        typename M::acc_t acc[TQ];
+        int r0[TQ], c0[TQ];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = k0 + r0 + M::row(g, v);
-          acc[v] = (row < n && col < n) ? L[row + (long)col * ld] : (T)0;
+        for (int q = 0; q < TQ; ++q) {
+          int bi = 0, bj = 0;
+          if (t0 + q < ntile) tri_rc(t0 + q, bi, bj);
+          r0[q] = kb + 16 * bi;  // panel-relative
+          c0[q] = kb + 16 * bj;
+          const int col = k0 + c0[q] + l16;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = k0 + r0[q] + M::row(g, v);
+            acc[q][v] = (t0 + q < ntile && row < n && col < n) ? L[row + (long)col * ld] : (T)0;
+          }
         }
         for (int kk = 0; kk < kk4; kk += 4) {
-          const T av = P[(kk + g) * S + r0 + l16];
-          const T bv = P[(kk + g) * S + c0 + l16];
-          acc = M::mma(-av, bv, acc);
+#pragma unroll
+          for (int q = 0; q < TQ; ++q) {
+            const T av = P[(kk + g) * S + r0[q] + l16];
+            const T bv = P[(kk + g) * S + c0[q] + l16];
+            acc[q] = M::mma(-av, bv, acc[q]);
+          }
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int row = k0 + r0 + M::row(g, v);
-          if (row < n && col < n) L[row + (long)col * ld] = acc[v];
+        for (int q = 0; q < TQ; ++q) {
+          const int col = k0 + c0[q] + l16;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = k0 + r0[q] + M::row(g, v);
+            if (t0 + q < ntile && row < n && col < n) L[row + (long)col * ld] = acc[q][v];
+          }
         }
       }
     }
@@ -275,36 +289,52 @@ __device__ void wg_potrs(const T *L, int ld, int n, const T *Linv, double *X, in
 }
 
 // R = B - A X with the FP64 matrix (full storage, symmetric: column c is row
-// c), B the stacked identity; returns max |R|.  One wave per column.
+// c), B the stacked identity; returns max |R|.  Each wave takes four columns
+// at a time (lanes over rows, so the loads of four columns are in flight
+// together), then reduces each column's three dots across the wave.
 template <int NT>
 __device__ double wg_residual(const double *A, int ld, int n, const double *X, double *R, int S,
                               double *red) {
+  constexpr int CW = 4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double rmax = 0.0;
-  for (int c = wave; c < n; c += NT / 64) {
-    const double *a = A + (long)c * ld;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int c0 = CW * wave; c0 < n; c0 += CW * (NT / 64)) {
+    double s[CW][3];
+#pragma unroll
+    for (int q = 0; q < CW; ++q) s[q][0] = s[q][1] = s[q][2] = 0.0;
     for (int r = lane; r < n; r += 64) {
-      const double v = a[r];
-      s0 += v * X[r];
-      s1 += v * X[S + r];
-      s2 += v * X[2 * S + r];
+      const double x0 = X[r], x1 = X[S + r], x2 = X[2 * S + r];
+#pragma unroll
+      for (int q = 0; q < CW; ++q) {
+        if (c0 + q < n) {
+          const double v = A[r + (long)(c0 + q) * ld];
+          s[q][0] += v * x0;
+          s[q][1] += v * x1;
+          s[q][2] += v * x2;
+        }
+      }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      s0 += __shfl_xor(s0, o, 64);
-      s1 += __shfl_xor(s1, o, 64);
-      s2 += __shfl_xor(s2, o, 64);
+#pragma unroll
+    for (int q = 0; q < CW; ++q) {
+      for (int o = 32; o > 0; o >>= 1) {
+        s[q][0] += __shfl_xor(s[q][0], o, 64);
+        s[q][1] += __shfl_xor(s[q][1], o, 64);
+        s[q][2] += __shfl_xor(s[q][2], o, 64);
+      }
+      const int c = c0 + q;
+      if (c < n) {
+        const int b = c % 3;
+        const double r0 = (b == 0 ? 1.0 : 0.0) - s[q][0];
+        const double r1 = (b == 1 ? 1.0 : 0.0) - s[q][1];
+        const double r2 = (b == 2 ? 1.0 : 0.0) - s[q][2];
+        if (lane == 0) {
+          R[c] = r0;
+          R[S + c] = r1;
+          R[2 * S + c] = r2;
+        }
+        rmax = fmax(rmax, fmax(fabs(r0), fmax(fabs(r1), fabs(r2))));
+      }
     }
-    const int b = c % 3;
-    const double r0 = (b == 0 ? 1.0 : 0.0) - s0;
-    const double r1 = (b == 1 ? 1.0 : 0.0) - s1;
-    const double r2 = (b == 2 ? 1.0 : 0.0) - s2;
-    if (lane == 0) {
-      R[c] = r0;
-      R[S + c] = r1;
-      R[2 * S + c] = r2;
-    }
-    rmax = fmax(rmax, fmax(fabs(r0), fmax(fabs(r1), fabs(r2))));
   }
   return wg_max<NT>(rmax, red);
 }
@@ -317,56 +347,136 @@ __device__ inline void set_identity_rhs(double *X, int n, int S, int NT) {
   }
 }
 
-// Householder reduction of the symmetric n x n FP64 matrix A (full storage,
-// overwritten) to tridiagonal d, e (dsytd2 'L' steps, unblocked).
-template <int NT>
+// Reflector scalars (LAPACK dlarfg without its rescaling loop): from the head
+// element alp and xn2 = |rest|^2, tau, beta and the scale of the rest.
+__device__ inline void b_reflector(double xn2, double alp, double &tau, double &beta, double &sc) {
+  if (xn2 == 0.0) {
+    tau = 0.0;
+    beta = alp;
+    sc = 0.0;
+  } else {
+    beta = -copysign(sqrt(alp * alp + xn2), alp);
+    tau = (beta - alp) / beta;
+    sc = 1.0 / (alp - beta);
+  }
+}
+
+// One pass over the lower triangle of A[c0:n, c0:n]: if upd, first apply the
+// previous step's rank-2 update A -= v w^T + w v^T (and store it); then
+// accumulate y = A vn as per-wave row partials pw[wave S + r] (entries on and
+// left of the diagonal) and column sums pc[c] (entries strictly below it).
+// Rows go in chunks of 64 KM; lane l owns rows rb + l + 64 k of a chunk and
+// wave (c - c0) mod NW owns column c, so every partial has one owner and a
+// fixed summation order.
+template <int NT, int KM>
+__device__ void trd_pass(double *A, int ld, int n, int c0, bool upd, const double *v,
+                         const double *w, const double *vn, double *pw, double *pc, int S) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int rb = c0; rb < n; rb += 64 * KM) {
+    const int re = min(n, rb + 64 * KM);
+    // the lane's rows of v, w, vn stay in registers for the whole chunk
+    double racc[KM], vr[KM], wr[KM], vnr[KM];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      const int r = rb + lane + 64 * k;
+      racc[k] = 0.0;
+      vr[k] = (upd && r < re) ? v[r] : 0.0;
+      wr[k] = (upd && r < re) ? w[r] : 0.0;
+      vnr[k] = (r < re) ? vn[r] : 0.0;
+    }
+    for (int c = c0 + wave; c < re; c += NW) {
+      double *col = A + (long)c * ld;
+      const double vc = upd ? v[c] : 0.0, wc = upd ? w[c] : 0.0, vnc = vn[c];
+      double cs = 0.0;
+#pragma unroll
+      for (int k = 0; k < KM; ++k) {
+        const int r = rb + lane + 64 * k;
+        if (r >= c && r < re) {
+          double t = col[r];
+          if (upd) {
+            t -= vr[k] * wc + wr[k] * vc;
+            col[r] = t;
+          }
+          racc[k] += t * vnc;
+          if (r != c) cs += t * vnr[k];
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) cs += __shfl_xor(cs, o, 64);
+      if (lane == 0) pc[c] = (c >= rb ? 0.0 : pc[c]) + cs;
+    }
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      const int r = rb + lane + 64 * k;
+      if (r < re) pw[wave * S + r] = racc[k];
+    }
+  }
+  __syncthreads();
+}
+
+// Householder reduction of the symmetric n x n FP64 matrix A (lower triangle
+// read and updated in place) to tridiagonal d, e (the dsytd2 'L' steps).  The
+// update of step j and the product of step j + 1 share one pass over the
+// trailing triangle: column j + 1 is updated first, its reflector vn formed,
+// then trd_pass updates the rest and accumulates A vn in the same sweep.
+template <int NT, int KM>
 __device__ void wg_sytrd(double *A, int ld, int n, double *d, double *e, double *v, double *w,
-                         double *red) {
+                         double *vn, double *pw, double *pc, int S, double *red) {
+  constexpr int NW = NT / 64;
   const int tid = threadIdx.x;
-  for (int j = 0; j < n - 1; ++j) {
-    const int m = n - j - 1;
-    __syncthreads();
-    const double *x = A + (j + 1) + (long)j * ld;
-    double sg = 0.0;
-    for (int r = tid; r < m; r += NT) {
-      const double xr = x[r];
-      v[r] = xr;
-      if (r > 0) sg += xr * xr;
-    }
-    sg = wg_sum<NT>(sg, red);
-    const double alpha = v[0];
-    if (tid == 0) d[j] = A[j + (long)j * ld];
-    if (sg == 0.0) {
-      if (tid == 0) e[j] = alpha;
-      continue;
-    }
-    const double beta = -copysign(sqrt(alpha * alpha + sg), alpha);
-    const double tau = (beta - alpha) / beta;
-    const double scal = 1.0 / (alpha - beta);
-    if (tid == 0) e[j] = beta;
-    __syncthreads();
-    for (int r = tid; r < m; r += NT) v[r] = (r == 0) ? 1.0 : v[r] * scal;
-    __syncthreads();
+  // step 0: reflector of column 0, y = A[1:, 1:] v
+  double sg = 0.0;
+  for (int r = 2 + tid; r < n; r += NT) {
+    const double x = A[r];
+    sg += x * x;
+  }
+  sg = wg_sum<NT>(sg, red);
+  double tau, beta, sc;
+  b_reflector(sg, n > 1 ? A[1] : 0.0, tau, beta, sc);
+  if (tid == 0) {
+    d[0] = A[0];
+    if (n > 1) e[0] = beta;
+  }
+  for (int r = 1 + tid; r < n; r += NT) v[r] = (r == 1) ? 1.0 : sc * A[r];
+  __syncthreads();
+  if (n > 1) trd_pass<NT, KM>(A, ld, n, 1, false, v, w, v, pw, pc, S);
+  for (int j = 0; j + 1 < n; ++j) {
+    // here v (rows j+1..n-1) is step j's reflector; p = A_j v in pw / pc
     double pv = 0.0;
-    double *A22 = A + (j + 1) + (long)(j + 1) * ld;
-    for (int r = tid; r < m; r += NT) {
-      double s = 0.0;
-      for (int c = 0; c < m; ++c) s += A22[r + (long)c * ld] * v[c];
-      const double p = tau * s;
+    for (int r = j + 1 + tid; r < n; r += NT) {
+      double p = pc[r];
+      for (int q = 0; q < NW; ++q) p += pw[q * S + r];
+      p *= tau;
       w[r] = p;
       pv += p * v[r];
     }
     const double K = 0.5 * tau * wg_sum<NT>(pv, red);
-    for (int r = tid; r < m; r += NT) w[r] -= K * v[r];
+    for (int r = j + 1 + tid; r < n; r += NT) w[r] -= K * v[r];
     __syncthreads();
-    for (int r = tid; r < m; r += NT) {
-      const double vr = v[r], wr = w[r];
-      double *a = A22 + r;
-      for (int c = 0; c < m; ++c) a[(long)c * ld] -= vr * w[c] + wr * v[c];
+    // column j + 1 after step j's update (rows j+1..n-1), kept in vn
+    const int c1 = j + 1;
+    const double *col = A + (long)c1 * ld;
+    const double v1 = v[c1], w1 = w[c1];
+    double s2 = 0.0;
+    for (int r = c1 + tid; r < n; r += NT) {
+      const double u = col[r] - (v[r] * w1 + w[r] * v1);
+      vn[r] = u;
+      if (r >= c1 + 2) s2 += u * u;
     }
+    s2 = wg_sum<NT>(s2, red);
+    if (tid == 0) d[c1] = vn[c1];
+    if (c1 == n - 1) break;
+    const double alp = vn[c1 + 1];
+    b_reflector(s2, alp, tau, beta, sc);
+    if (tid == 0) e[c1] = beta;
+    __syncthreads();
+    for (int r = c1 + 1 + tid; r < n; r += NT) vn[r] = (r == c1 + 1) ? 1.0 : sc * vn[r];
+    __syncthreads();
+    trd_pass<NT, KM>(A, ld, n, c1 + 1, true, v, w, vn, pw, pc, S);
+    double *t = v;
+    v = vn;
+    vn = t;
   }
-  __syncthreads();
-  if (tid == 0) d[n - 1] = A[(n - 1) + (long)(n - 1) * ld];
   __syncthreads();
 }
 
@@ -605,17 +715,16 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         const double r = exact_dist(dx, dy, dz);
         if (r <= a.rc_cfdm) lr_tensor_add64(dx, dy, dz, r, rv[i] + rv[j], a.beta, pref, acc);
       }
+      // lower blocks only: the tridiagonalisation reads the lower triangle
       for (int cc = 0; cc < 3; ++cc)
-        for (int rr = 0; rr < 3; ++rr) {
-          const double v = acc[idx6[rr][cc]];
-          A64[(3 * i + rr) + (long)(3 * j + cc) * ld] = v;
-          A64[(3 * j + cc) + (long)(3 * i + rr) * ld] = v;
-        }
+        for (int rr = 0; rr < 3; ++rr)
+          A64[(3 * i + rr) + (long)(3 * j + cc) * ld] = acc[idx6[rr][cc]];
     }
     __syncthreads();
-    double *dv = (double *)(lds + lay.panel);
-    double *ev = dv + S, *e2 = ev + S, *vv = e2 + S, *ww = vv + S, *lam = ww + S;
-    wg_sytrd<NT>(A64, ld, n, dv, ev, vv, ww, red);
+    double *pw = (double *)(lds + lay.panel);
+    double *dv = R + S, *ev = R + 2 * S;
+    wg_sytrd<NT, 4>(A64, ld, n, dv, ev, X, X + S, X + 2 * S, pw, R, S, red);
+    double *e2 = pw, *lam = pw + S;
     wg_tri_eigvals<NT>(n, dv, ev, e2, lam, red);
     double s = 0.0, c = 0.0, neg = 0.0;
     for (int k = tid; k < n; k += NT) {

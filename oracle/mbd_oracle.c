@@ -672,6 +672,57 @@ static int or_scs_one(int N, const double *xyz, const double *hirsh,
   return info;
 }
 
+int or_scs_all(int N, const double *xyz, const double *hirsh,
+               const double *alpha0, const double *c6_0, const double *r0,
+               const double *lat, int periodic, const int *vacuum, double beta,
+               double rc_scs_ang, double *alpha_eff_out, double *coupled_pol_out,
+               double *molpol_eig_out, double *c6_mol_out, double *c6_eff_out,
+               int *info_out) {
+  const int n = 3 * N;
+  double zero_lat[9] = {0};
+  if (!periodic || !lat) lat = zero_lat;
+  int scs_img[3], sc[3], cf_img[3];
+  or_image_ranges(lat, periodic, vacuum, rc_scs_ang, 100.0, 25.0, scs_img, sc,
+                  cf_img);
+  double om[21], wt[21];
+  or_casimir_grid(om, wt);
+  double *A = (double *)malloc(sizeof(double) * (size_t)n * n);
+  double *X = (double *)malloc(sizeof(double) * (size_t)n * 3);
+  double *ascs = (double *)malloc(sizeof(double) * (size_t)N);
+  double *cpol = (double *)malloc(sizeof(double) * 20 * (size_t)N);
+  int info = 0;
+  double mol_c6 = 0.0;
+  for (int k = 0; k <= 20; ++k) { /* UTILS.f90:239-281 */
+    double P[9], ev[3];
+    int inf = or_scs_one(N, xyz, hirsh, alpha0, c6_0, r0, lat, scs_img,
+                         rc_scs_ang, beta, om[k], A, X, ascs, P);
+    if (inf != 0 && info == 0) info = inf;
+    if (k == 0)
+      memcpy(alpha_eff_out, ascs, sizeof(double) * (size_t)N);
+    else
+      for (int i = 0; i < N; ++i) cpol[(k - 1) + 20 * (size_t)i] = ascs[i];
+    or_sym3_eigvals(P, ev);
+    double iso = (ev[0] + ev[1] + ev[2]) / 3.0;
+    if (k >= 1) mol_c6 += wt[k] * (iso * iso);
+    if (molpol_eig_out)
+      for (int c = 0; c < 3; ++c) molpol_eig_out[3 * k + c] = ev[c];
+  }
+  for (int i = 0; i < N; ++i) { /* UTILS.f90:302-315 */
+    double s = 0.0;
+    for (int k = 1; k <= 20; ++k) {
+      double a = cpol[(k - 1) + 20 * (size_t)i];
+      s += wt[k] * (a * a);
+    }
+    c6_eff_out[i] = s * OR_THREE_OVER_PI;
+  }
+  if (coupled_pol_out)
+    memcpy(coupled_pol_out, cpol, sizeof(double) * 20 * (size_t)N);
+  if (c6_mol_out) *c6_mol_out = OR_THREE_OVER_PI * mol_c6;
+  if (info_out) *info_out = info;
+  free(A); free(X); free(ascs); free(cpol);
+  return 0;
+}
+
 int or_mbd_run(int N, const double *xyz, const double *hirsh,
                const double *alpha0, const double *c6_0, const double *r0,
                const double *lat, int periodic, const int *vacuum,
@@ -687,41 +738,13 @@ int or_mbd_run(int N, const double *xyz, const double *hirsh,
   int scs_img[3], sc[3], cf_img[3];
   or_image_ranges(lat, periodic, vacuum, rc_scs_ang, rc_cfdm_ang, rc_sc_ang,
                   scs_img, sc, cf_img);
-  double om[21], wt[21];
-  or_casimir_grid(om, wt);
-  double *A = (double *)malloc(sizeof(double) * (size_t)n * n);
-  double *X = (double *)malloc(sizeof(double) * (size_t)n * 3);
   double *aeff = (double *)malloc(sizeof(double) * (size_t)N);
-  double *cpol = (double *)malloc(sizeof(double) * 20 * (size_t)N);
-  double *ascs = (double *)malloc(sizeof(double) * (size_t)N);
-  int info = 0;
-  double mol_c6 = 0.0;
-  for (int k = 0; k <= 20; ++k) { /* UTILS.f90:239-281 */
-    double P[9], ev[3];
-    int inf = or_scs_one(N, xyz, hirsh, alpha0, c6_0, r0, lat, scs_img,
-                         rc_scs_ang, beta, om[k], A, X, ascs, P);
-    if (inf != 0 && info == 0) info = inf;
-    if (k == 0)
-      memcpy(aeff, ascs, sizeof(double) * (size_t)N);
-    else
-      for (int i = 0; i < N; ++i) cpol[(k - 1) + 20 * (size_t)i] = ascs[i];
-    or_sym3_eigvals(P, ev);
-    double iso = (ev[0] + ev[1] + ev[2]) / 3.0;
-    if (k >= 1) mol_c6 += wt[k] * (iso * iso);
-    if (molpol_eig_out)
-      for (int c = 0; c < 3; ++c) molpol_eig_out[3 * k + c] = ev[c];
-  }
   double *c6e = (double *)malloc(sizeof(double) * (size_t)N);
   double *omg = (double *)malloc(sizeof(double) * (size_t)N);
   double *ref = (double *)malloc(sizeof(double) * (size_t)N);
-  for (int i = 0; i < N; ++i) { /* UTILS.f90:302-315 */
-    double s = 0.0;
-    for (int k = 1; k <= 20; ++k) {
-      double a = cpol[(k - 1) + 20 * (size_t)i];
-      s += wt[k] * (a * a);
-    }
-    c6e[i] = s * OR_THREE_OVER_PI;
-  }
+  or_scs_all(N, xyz, hirsh, alpha0, c6_0, r0, lat, periodic, vacuum, beta,
+             rc_scs_ang, aeff, coupled_pol_out, molpol_eig_out, c6_mol_out, c6e,
+             info_out);
   for (int i = 0; i < N; ++i) { /* UTILS.f90:731-741 */
     ref[i] = r0[i] * pow(aeff[i] / alpha0[i], OR_OTD);
     omg[i] = (4.0 / 3.0) * c6e[i] / (aeff[i] * aeff[i]);
@@ -744,18 +767,13 @@ int or_mbd_run(int N, const double *xyz, const double *hirsh,
   }
   *e_mbd = (0.5 * e_int) / (double)ncell - 0.5 * e_nonint;
   if (alpha_eff_out) memcpy(alpha_eff_out, aeff, sizeof(double) * (size_t)N);
-  if (coupled_pol_out)
-    memcpy(coupled_pol_out, cpol, sizeof(double) * 20 * (size_t)N);
-  if (c6_mol_out) *c6_mol_out = OR_THREE_OVER_PI * mol_c6;
   if (c6_eff_out) memcpy(c6_eff_out, c6e, sizeof(double) * (size_t)N);
   if (omega_out) memcpy(omega_out, omg, sizeof(double) * (size_t)N);
   if (reff_out) memcpy(reff_out, ref, sizeof(double) * (size_t)N);
   if (supercell_out)
     for (int k = 0; k < 3; ++k) supercell_out[k] = sc[k];
   if (nimag_out) *nimag_out = nimag;
-  if (info_out) *info_out = info;
-  free(A); free(X); free(aeff); free(cpol); free(ascs);
-  free(c6e); free(omg); free(ref); free(C); free(ev);
+  free(aeff); free(c6e); free(omg); free(ref); free(C); free(ev);
   return 0;
 }
 

@@ -79,6 +79,16 @@ void or_tridiag_eigvals(int n, const double *d, const double *e, double *w);
  * is read from P (column-major), as DSYEV('V','U',3) at UTILS.f90:273. */
 void or_sym3_eigvals(const double *P, double *w);
 
+/* The SCS half of or_mbd_run: alpha_SCS at all 21 points, C6 sums
+ * (UTILS.f90:239-315).  alpha_eff[N], c6_eff[N] required; coupled_pol[20 N]
+ * (frequency fastest), molpol_eig[63], c6_mol, info optional. */
+int or_scs_all(int N, const double *xyz, const double *hirsh,
+               const double *alpha0, const double *c6_0, const double *r0,
+               const double *lat, int periodic, const int *vacuum, double beta,
+               double rc_scs_ang, double *alpha_eff_out, double *coupled_pol_out,
+               double *molpol_eig_out, double *c6_mol_out, double *c6_eff_out,
+               int *info_out);
+
 /* Whole MBD@rsSCS evaluation (UTILS.f90:153-349 + 700-879).
  * Inputs: coordinates xyz[3*i + c] in bohr, Hirshfeld ratios, free-atom
  * (alpha0, C6_0, R0) per atom, lattice (bohr) when periodic, vacuum flags,

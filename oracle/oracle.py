@@ -64,6 +64,8 @@ def lib():
                                   ctypes.c_double]
                                  + [_D] * 8 + [_I, _I, _I])
         L.or_mbd_run.restype = ctypes.c_int
+        L.or_scs_all.argtypes = [ctypes.c_int, _D, _D, _D, _D, _D, _D, ctypes.c_int, _I,
+                                 ctypes.c_double, ctypes.c_double, _D, _D, _D, _D, _D, _I]
         L.or_time_scs_frequency.argtypes = [ctypes.c_int, _D, _D, _D, _D, _D, _D, ctypes.c_int,
                                             _I, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.or_time_scs_frequency.restype = ctypes.c_double
@@ -228,6 +230,20 @@ def mbd_run(s: OracleSystem) -> OracleResult:
                      _p(om), _p(ref), _pi(sc), ctypes.byref(nimag), ctypes.byref(info))
     return OracleResult(e.value, aeff, cpol.reshape(N, 20), mp.reshape(21, 3), c6m.value,
                         c6e, om, ref, sc, nimag.value, info.value)
+
+
+def scs_all(s: OracleSystem):
+    """SCS half only: (alpha_eff, coupled_pol (N, 20), c6_eff, c6_mol, info)."""
+    N = s.n_atoms
+    vac = np.array([int(v) for v in s.vacuum], dtype=np.int32)
+    aeff, c6e = np.zeros(N), np.zeros(N)
+    cpol = np.zeros(20 * N)
+    c6m, info = ctypes.c_double(), ctypes.c_int()
+    lat = np.ascontiguousarray(s.lattice, dtype=np.float64)
+    lib().or_scs_all(N, _p(s.xyz), _p(s.hirsh), _p(s.alpha0), _p(s.c6_0), _p(s.r0), _p(lat),
+                     int(s.periodic), _pi(vac), s.beta, s.rc_scs, _p(aeff), _p(cpol), None,
+                     ctypes.byref(c6m), _p(c6e), ctypes.byref(info))
+    return aeff, cpol.reshape(N, 20), c6e, c6m.value, info.value
 
 
 def relay_matrix(s: OracleSystem, freq_index: int) -> np.ndarray:
