@@ -86,6 +86,7 @@ def traffic_probe(args):
     """Workload of the hardware-counter runs: one 1 GiB copy with the plain
     copy kernel (the known-size reference), then one solve of the headline
     form.  Meant to run under rocprofv3 --pmc; prints progress only."""
+    os.environ["MBD_TRD_GRAPH"] = "0"  # eager launches: one counter record per dispatch
     from mbd_amd import engine, generators, learned
     ms = engine.device_copy(1 << 30)
     log(f"traffic-probe: copy 1 GiB in {ms:.3f} ms")
@@ -107,11 +108,8 @@ def batched_line(steps, warmup):
     mols, st = generators.chain_molecules()
     eng = engine.MbdEngine(0)
     eng.batch_set(mols, st)
-    best, times = eng.batch_tune("mixed", reps=2)
     out = {"workload": "c4_chains_512: 512 non-periodic chains, 8-256 atoms, all 21 points",
-           "molecules": len(mols), "atoms": int(sum(m.n_atoms for m in mols)),
-           "tuned": {"index": best, **engine.batch_config(best)},
-           "tuner_ms": [round(t, 3) for t in times]}
+           "molecules": len(mols), "atoms": int(sum(m.n_atoms for m in mols))}
     res = {}
     for mode in ("mixed", "fp64"):
         for _ in range(warmup):
@@ -123,6 +121,8 @@ def batched_line(steps, warmup):
             dev += r["diag"]["t_ms"]
         dt = (time.perf_counter() - t0) / steps
         res[mode] = eng.batch_run(mode)
+        out["launch_shape"] = {"index": res[mode]["diag"]["config"],
+                               **engine.batch_config(res[mode]["diag"]["config"])}
         out[mode] = {"ms_per_batch": 1e3 * dt, "device_ms_per_batch": dev / steps,
                      "molecules_per_s": len(mols) / dt,
                      "fallbacks": res[mode]["diag"]["fallbacks"],

@@ -221,6 +221,10 @@ int mbd_batch_set(mbd_handle *h, int nmol, const int *offsets, const double *xyz
  * not eligible); *best the chosen index. */
 int mbd_batch_tune(mbd_handle *h, int mode, int reps, double *times_ms, int *best);
 
+/* Use candidate `index` for later mbd_batch_run calls (-1: the default
+ * shape); MBD_ERR_TOOBIG if it cannot hold the batch's largest molecule. */
+int mbd_batch_set_config(mbd_handle *h, int index);
+
 /* Candidate table of the autotuner. */
 int mbd_batch_config(int index, int *nb, int *nt, int *per_cu);
 

@@ -94,6 +94,7 @@ def lib():
     L.mbd_batch_set.argtypes = [h, ctypes.c_int, _I, _D, _D, _D, _D, _D, ctypes.c_double,
                                 ctypes.c_double, ctypes.c_double]
     L.mbd_batch_tune.argtypes = [h, ctypes.c_int, ctypes.c_int, _D, _I]
+    L.mbd_batch_set_config.argtypes = [h, ctypes.c_int]
     L.mbd_batch_config.argtypes = [ctypes.c_int, _I, _I, _I]
     L.mbd_batch_run.argtypes = [h, ctypes.c_int, _D, _D, _D, _D, _D, _I,
                                 ctypes.POINTER(MbdBatchDiag)]

@@ -191,6 +191,10 @@ struct mbd_handle {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     int exec_cfg[2] = {-1, -1};
   } bt;
+  // tridiagonalisation replayed from a captured graph (36k small launches at
+  // n = 12288); rebuilt when the size or any buffer address changes
+  hipGraphExec_t trd_exec = nullptr;
+  std::vector<const void *> trd_key;
   // instrumentation
   bool prof_on = false;
   Profiler prof;
@@ -466,7 +470,35 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
   wk.alpha = h->alph.p;
   wk.pa = h->pa.p;
   wk.pb = h->pb.p;
-  tridiagonalize(s, (int)nm, h->C.p, nm, wk);
+  const char *g = getenv("MBD_TRD_GRAPH");
+  if (h->prof_on || (g && g[0] == '0')) {
+    tridiagonalize(s, (int)nm, h->C.p, nm, wk);
+  } else {
+    const std::vector<const void *> key = {(const void *)(intptr_t)nm, h->C.p, h->VW.p,
+                                           h->tau.p, h->dd.p, h->ee.p, h->alph.p, h->pa.p,
+                                           h->pb.p, h->py.p, h->qred.p};
+    if (!h->trd_exec || h->trd_key != key) {
+      if (h->trd_exec) (void)hipGraphExecDestroy(h->trd_exec);
+      h->trd_exec = nullptr;
+      h->trd_key.clear();
+      tridiag_prepare();
+      hipGraph_t gr = nullptr;
+      MBD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      try {
+        tridiagonalize(s, (int)nm, h->C.p, nm, wk);
+      } catch (const HipError &) {
+        (void)hipStreamEndCapture(s, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        throw;
+      }
+      MBD_HIP(hipStreamEndCapture(s, &gr));
+      const hipError_t ie = hipGraphInstantiate(&h->trd_exec, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      MBD_HIP(ie);
+      h->trd_key = key;
+    }
+    MBD_HIP(hipGraphLaunch(h->trd_exec, s));
+  }
   h->lam.ensure(nm);
   h->e2.ensure(nm);
   h->bounds.ensure(4);
@@ -572,6 +604,7 @@ int mbd_destroy(mbd_handle *h) {
   h->cell.release();
   h->nneg.release();
   batch_release(h);
+  if (h->trd_exec) (void)hipGraphExecDestroy(h->trd_exec);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return 0;
@@ -1417,6 +1450,15 @@ int mbd_batch_set(mbd_handle *h, int nmol, const int *offsets, const double *xyz
   } catch (const HipError &e) {
     return fail(h, e);
   }
+  return 0;
+}
+
+int mbd_batch_set_config(mbd_handle *h, int index) {
+  if (!h) return -1;
+  if (index < -1 || index >= kBatchNumConfigs) return -2;
+  if (!h->bt.nmol) return MBD_ERR_NOSYS;
+  if (index >= 0 && !batch_lds_bytes(kBatchConfigs[index], h->bt.nmax)) return MBD_ERR_TOOBIG;
+  h->bt.cfg = index;
   return 0;
 }
 

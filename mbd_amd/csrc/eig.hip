@@ -18,6 +18,9 @@
 // lower tiles) applies A22 -= [V W][W V]^T.  The last <= kTrdTail rows are
 // reduced by one workgroup in LDS.
 // Eigenvalues: Sturm-count bisection, one thread per eigenvalue index.
+#include <algorithm>
+#include <cstdlib>
+
 #include "common.h"
 #include "kernels.h"
 #include "profile.h"
@@ -69,11 +72,12 @@ __device__ inline void reflector(double xn2, double alp, double &tau, double &be
 }  // namespace
 
 // 64 rows per workgroup, 4 threads per row (the p-sums split by p mod 4).
-__global__ __launch_bounds__(256) void k_trd_colupd(
+__device__ __forceinline__ void trd_colupd_body(
     int n, double *__restrict__ A, long lda, double *__restrict__ VW, long ldv,
     int j0, int j, const double *__restrict__ tau, double *__restrict__ alpha_pan,
     const double *__restrict__ pb_part, int npb, double *__restrict__ pa, int pald,
-    double *__restrict__ d) {
+    double *__restrict__ d, int bx) {
+  __syncthreads();
   const int i = j - j0;
   __shared__ double al[NBT], vj[NBT], wj[NBT];
   __shared__ double red[4][64];
@@ -88,7 +92,7 @@ __global__ __launch_bounds__(256) void k_trd_colupd(
     if (lane == 0) {
       const double a = -0.5 * tau[j - 1] * acc;
       al[i - 1] = a;
-      if (blockIdx.x == 0) alpha_pan[i - 1] = a;
+      if (bx == 0) alpha_pan[i - 1] = a;
     }
   }
   for (int p = t; p < i - 1; p += blockDim.x) al[p] = alpha_pan[p];
@@ -99,7 +103,7 @@ __global__ __launch_bounds__(256) void k_trd_colupd(
     wj[p] = W[j + (long)p * ldv] + al[p] * v;
   }
   __syncthreads();
-  const int r = j + blockIdx.x * 64 + lane;
+  const int r = j + bx * 64 + lane;
   double s = 0.0;
   if (r < n) {
 #pragma unroll 4
@@ -121,12 +125,13 @@ __global__ __launch_bounds__(256) void k_trd_colupd(
     sv[lane] = (r >= j + 2 && r < n) ? val : 0.0;
   }
   __syncthreads();
-  // partials q < i: W^T a, i <= q < 2i: V^T a, q = 2i: |a|^2; one wave per q
+  // partials q < i: W^T a, i <= q < 2i: V^T a, q = 2i: |a|^2; each wave
+  // takes 16 quantities at a time so their loads are in flight together
   const double x = sv[lane];
-  for (int q0 = part * 4; q0 <= 2 * i; q0 += 16) {
-    double acc[4];
+  for (int q0 = part * 16; q0 <= 2 * i; q0 += 64) {
+    double acc[16];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 16; ++u) {
       const int q = q0 + u;
       acc[u] = 0.0;
       if (q == 2 * i) {
@@ -137,24 +142,33 @@ __global__ __launch_bounds__(256) void k_trd_colupd(
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc[u] = wave_sum(acc[u]);
+    for (int u = 0; u < 16; ++u) acc[u] = wave_sum(acc[u]);
     if (lane == 0) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (q0 + u <= 2 * i) pa[(long)(q0 + u) * pald + blockIdx.x] = acc[u];
+      for (int u = 0; u < 16; ++u)
+        if (q0 + u <= 2 * i) pa[(long)(q0 + u) * pald + bx] = acc[u];
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_trd_colupd(
+    int n, double *__restrict__ A, long lda, double *__restrict__ VW, long ldv,
+    int j0, int j, const double *__restrict__ tau, double *__restrict__ alpha_pan,
+    const double *__restrict__ pb_part, int npb, double *__restrict__ pa, int pald,
+    double *__restrict__ d) {
+  trd_colupd_body(n, A, lda, VW, ldv, j0, j, tau, alpha_pan, pb_part, npb, pa, pald, d, blockIdx.x);
 }
 
 // y = A22 v on the lower triangle, tile (I, J), I >= J, of the trailing
 // matrix starting at row/column o = j + 1.  Writes the tile's row dots to
 // Py[I][J] and its column dots to Py[J][I] (on the diagonal, both to Py[I][I]).
-__global__ __launch_bounds__(256) void k_trd_symv(int n, const double *__restrict__ A,
+__device__ __forceinline__ void trd_symv_body(int n, const double *__restrict__ A,
                                                   long lda, int j0, int j,
                                                   const double *__restrict__ pa, int pald,
                                                   int npa, int nbt,
                                                   double *__restrict__ Py,
-                                                  double *__restrict__ qred) {
+                                                  double *__restrict__ qred, int bx, int nb) {
+  __syncthreads();
   __shared__ double vI[TS], vJ[TS];
   __shared__ double rowp[4][TS];
   __shared__ double colp[TS];
@@ -162,12 +176,13 @@ __global__ __launch_bounds__(256) void k_trd_symv(int n, const double *__restric
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int i = j - j0;
   int I, J;
-  tri_index(blockIdx.x, I, J);
+  tri_index(bx, I, J);
   const int o = j + 1;
   const double *colj = A + (long)j * lda;
-  if (blockIdx.x == 0) {
-    // reduce every colupd partial once for k_trd_wcol: qred[0..2i]
-    for (int q0 = wave * 8; q0 <= 2 * i; q0 += 32) {
+  {
+    // reduce every colupd partial once for k_trd_wcol: qred[0..2i], eight
+    // quantities per wave, spread over the workgroups of the launch
+    for (int q0 = (bx * 4 + wave) * 8; q0 <= 2 * i; q0 += nb * 32) {
       double acc[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) acc[u] = 0.0;
@@ -257,13 +272,23 @@ __global__ __launch_bounds__(256) void k_trd_symv(int n, const double *__restric
   }
 }
 
+__global__ __launch_bounds__(256) void k_trd_symv(int n, const double *__restrict__ A,
+                                                  long lda, int j0, int j,
+                                                  const double *__restrict__ pa, int pald,
+                                                  int npa, int nbt,
+                                                  double *__restrict__ Py,
+                                                  double *__restrict__ qred) {
+  trd_symv_body(n, A, lda, j0, j, pa, pald, npa, nbt, Py, qred, blockIdx.x, gridDim.x);
+}
+
 // W and V columns for rows o..n-1 from the symv partials; 64 rows per
 // workgroup, 4 threads per row.
-__global__ __launch_bounds__(256) void k_trd_wcol(
+__device__ __forceinline__ void trd_wcol_body(
     int n, const double *__restrict__ A, long lda, double *__restrict__ VW, long ldv,
     int j0, int j, double *__restrict__ tau, double *__restrict__ e,
     const double *__restrict__ alpha_pan, const double *__restrict__ qred, int nbt,
-    const double *__restrict__ Py, double *__restrict__ pb_part) {
+    const double *__restrict__ Py, double *__restrict__ pb_part, int bx) {
+  __syncthreads();
   const int i = j - j0;
   __shared__ double q_s[PA_STRIDE];
   __shared__ double al[NBT], t1[NBT], t2[NBT];
@@ -282,7 +307,7 @@ __global__ __launch_bounds__(256) void k_trd_wcol(
     reflector(q_s[2 * i], colj[o], tv, beta, sc);
     tau_s = tv;
     sc_s = sc;
-    if (blockIdx.x == 0) {
+    if (bx == 0) {
       tau[j] = tv;
       e[j] = beta;
     }
@@ -298,19 +323,20 @@ __global__ __launch_bounds__(256) void k_trd_wcol(
     t2[p] = vh + sc * S2;
   }
   __syncthreads();
-  const int c = o + blockIdx.x * 64 + lane;
+  const int c = o + bx * 64 + lane;
   double part_sum = 0.0;
   if (c < n) {
     const int Ib = (c - o) / TS, cl = (c - o) % TS;
     const double *py = Py + (long)Ib * nbt * TS + cl;
-    double y = 0.0, y2 = 0.0;
-    int Jb = part;
-    for (; Jb + 4 < nbt; Jb += 8) {
-      y += py[(long)Jb * TS];
-      y2 += py[(long)(Jb + 4) * TS];
+    double ya[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) ya[u] = 0.0;
+    for (int Jb = part; Jb < nbt; Jb += 32) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (Jb + 4 * u < nbt) ya[u] += py[(long)(Jb + 4 * u) * TS];
     }
-    if (Jb < nbt) y += py[(long)Jb * TS];
-    y += y2;
+    const double y = ((ya[0] + ya[1]) + (ya[2] + ya[3])) + ((ya[4] + ya[5]) + (ya[6] + ya[7]));
     double corr = 0.0;
 #pragma unroll 4
     for (int p = part; p < i; p += 4) {
@@ -332,8 +358,16 @@ __global__ __launch_bounds__(256) void k_trd_wcol(
       contrib = wcc * vcc;
     }
     contrib = wave_sum(contrib);
-    if (lane == 0) pb_part[blockIdx.x] = contrib;
+    if (lane == 0) pb_part[bx] = contrib;
   }
+}
+
+__global__ __launch_bounds__(256) void k_trd_wcol(
+    int n, const double *__restrict__ A, long lda, double *__restrict__ VW, long ldv,
+    int j0, int j, double *__restrict__ tau, double *__restrict__ e,
+    const double *__restrict__ alpha_pan, const double *__restrict__ qred, int nbt,
+    const double *__restrict__ Py, double *__restrict__ pb_part) {
+  trd_wcol_body(n, A, lda, VW, ldv, j0, j, tau, e, alpha_pan, qred, nbt, Py, pb_part, blockIdx.x);
 }
 
 // Finalise the panel's W columns (W += alpha V), duplicate V after W so
@@ -441,6 +475,16 @@ __global__ __launch_bounds__(256) void k_trd_tail(int n, const double *A,
   if (t == 0) d[j0 + m - 1] = S[(m - 1) * ld + (m - 1)];
 }
 
+void tridiag_prepare() {
+  static bool attr = false;
+  if (!attr) {
+    const size_t mx = sizeof(double) * ((size_t)(kTrdTail + 1) * kTrdTail + 2 * kTrdTail);
+    MBD_HIP(hipFuncSetAttribute((const void *)k_trd_tail,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)mx));
+    attr = true;
+  }
+}
+
 void tridiagonalize(hipStream_t s, int n, double *A, long lda, TrdWork &wk) {
   int j0 = 0;
   while (n - j0 > kTrdTail) {
@@ -488,13 +532,7 @@ void tridiagonalize(hipStream_t s, int n, double *A, long lda, TrdWork &wk) {
   }
   const int m = n - j0;
   const size_t smem = sizeof(double) * ((size_t)(m + 1) * m + 2 * m);
-  static bool attr = false;
-  if (!attr) {
-    const size_t mx = sizeof(double) * ((size_t)(kTrdTail + 1) * kTrdTail + 2 * kTrdTail);
-    MBD_HIP(hipFuncSetAttribute((const void *)k_trd_tail,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)mx));
-    attr = true;
-  }
+  tridiag_prepare();
   ProfScope ps(K_TRD_TAIL, s, 0.0, (4.0 / 3.0) * m * (double)m * m);
   hipLaunchKernelGGL(k_trd_tail, dim3(1), dim3(256), smem, s, n, A, lda, j0,
                      wk.d, wk.e);

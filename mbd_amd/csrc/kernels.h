@@ -57,6 +57,8 @@ struct TrdWork {
   double *py;     // symv tile partials, nbt^2 x kTrdTile, nbt = ceil(n/kTrdTile)
 };
 void tridiagonalize(hipStream_t s, int n, double *A, long lda, TrdWork &wk);
+// one-time kernel attributes of tridiagonalize (call before capturing it)
+void tridiag_prepare();
 void tridiag_eigvals(hipStream_t s, int n, const double *d, const double *e,
                      double *e2, double *bounds, double *w);
 void energy(hipStream_t s, int n, const double *lam, int N,

@@ -226,6 +226,10 @@ class MbdEngine:
                     "mbd_batch_tune")
         return best.value, t
 
+    def batch_set_config(self, index: int):
+        """Force launch-shape candidate `index` (-1: default)."""
+        self._check(self._L.mbd_batch_set_config(self._h, int(index)), "mbd_batch_set_config")
+
     def batch_run(self, mode: str = "mixed", full: bool = True) -> dict:
         off = self._batch_off
         nmol, na = len(off) - 1, int(off[-1])
