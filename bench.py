@@ -125,6 +125,8 @@ def batched_line(steps, warmup):
                                **engine.batch_config(res[mode]["diag"]["config"])}
         out[mode] = {"ms_per_batch": 1e3 * dt, "device_ms_per_batch": dev / steps,
                      "molecules_per_s": len(mols) / dt,
+                     "workgroup_ms_by_phase": {k: round(v, 2) for k, v in
+                                               res[mode]["diag"]["phase_ms"].items()},
                      "fallbacks": res[mode]["diag"]["fallbacks"],
                      "max_iters": res[mode]["diag"]["max_iters"]}
         log(f"bench: batched {mode} {1e3 * dt:.2f} ms per 512 molecules")

@@ -205,6 +205,9 @@ typedef struct {
   int n_negative;      /* negative coupled-QHO eigenvalues, all molecules */
   double max_residual; /* largest final max|B - A X| */
   double t_ms;         /* device time of the replayed graph */
+  double phase_ms[6];  /* workgroup time summed over workgroups: relay build,
+                          factorisation, solves + refinement, C6 + coupled-QHO
+                          build, tridiagonalisation, bisection + energy */
 } mbd_batch_diag;
 
 /* offsets[nmol + 1] (offsets[0] = 0) split the atom arrays into molecules;
@@ -217,7 +220,7 @@ int mbd_batch_set(mbd_handle *h, int nmol, const int *offsets, const double *xyz
 
 /* Autotuner: times every candidate launch shape that fits the current batch
  * (reps launches each) and keeps the fastest for later mbd_batch_run calls.
- * times_ms[kBatchNumConfigs = 8] receives the average per candidate (-1 if
+ * times_ms[4] (one per candidate) receives the average per candidate (-1 if
  * not eligible); *best the chosen index. */
 int mbd_batch_tune(mbd_handle *h, int mode, int reps, double *times_ms, int *best);
 

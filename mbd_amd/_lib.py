@@ -33,7 +33,8 @@ class MbdBatchDiag(ctypes.Structure):
     _fields_ = [("config", ctypes.c_int), ("nb", ctypes.c_int), ("nt", ctypes.c_int),
                 ("grid", ctypes.c_int), ("fallbacks", ctypes.c_int), ("max_iters", ctypes.c_int),
                 ("n_failed", ctypes.c_int), ("n_negative", ctypes.c_int),
-                ("max_residual", ctypes.c_double), ("t_ms", ctypes.c_double)]
+                ("max_residual", ctypes.c_double), ("t_ms", ctypes.c_double),
+                ("phase_ms", ctypes.c_double * 6)]
 
 
 class LibraryMissing(RuntimeError):

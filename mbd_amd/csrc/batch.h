@@ -19,7 +19,10 @@ struct BatchConfig {
   int nt;
   int per_cu;
 };
-constexpr int kBatchNumConfigs = 8;
+constexpr int kBatchNumConfigs = 4;
+// workgroup time per phase: relay build, factorisation, solves and refinement,
+// C6 + coupled-QHO build, tridiagonalisation, bisection + energy
+constexpr int kBatchPhases = 6;
 extern const BatchConfig kBatchConfigs[kBatchNumConfigs];
 
 struct BatchArgs {
@@ -39,6 +42,7 @@ struct BatchArgs {
   int *stat;        // nmol x 4: info, n_negative, max iterations, fallbacks
   double *resid;    // nmol: largest final residual over the 21 points
   int *counter;     // [0]: next task; [1 + m]: finished points of molecule m
+  unsigned long long *phase;  // kBatchPhases summed workgroup ticks (wall clock)
 };
 
 // Dynamic LDS of the kernel for config c and largest matrix order nmax

@@ -32,9 +32,11 @@
 
 namespace mbd {
 
+// 256-thread workgroups only: at 512 threads a wave may hold 256 registers
+// and k_batch then spills to scratch (compiler resource report), so those
+// shapes are not built.
 const BatchConfig kBatchConfigs[kBatchNumConfigs] = {
-    {16, 256, 1}, {16, 256, 2}, {16, 512, 1}, {16, 512, 2},
-    {32, 256, 1}, {32, 256, 2}, {32, 512, 1}, {32, 512, 2}};
+    {16, 256, 1}, {16, 256, 2}, {32, 256, 1}, {32, 256, 2}};
 
 namespace {
 
@@ -570,6 +572,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     const int ld = n;
 
     // ---- SCS at grid point f --------------------------------------------
+    unsigned long long tk[4];
+    tk[0] = wall_clock64();
     const double w = a.omega[f];
     for (int i = tid; i < N; i += NT) {
       const double c6 = a.c60[o + i], a0 = a.a0[o + i], h = a.hirsh[o + i];
@@ -611,6 +615,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
         }
     }
     __syncthreads();
+    tk[1] = wall_clock64();
     int info = 0, iters = 0, fell = 0;
     double rmax = 0.0;
     bool done64 = false;
@@ -618,6 +623,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
       float *L = (float *)Lraw;
       info = wg_potrf<float, NB, NT>(L, ld, n, (float *)(lds + lay.panel), S,
                                      (float *)(lds + lay.z), (float *)Linv);
+      tk[2] = wall_clock64();
       if (info == 0) {
         set_identity_rhs(X, n, S, NT);
         wg_potrs<float, NB, NT>(L, ld, n, (const float *)Linv, X, S, tmp);
@@ -647,8 +653,11 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     }
     if (a.mode == 0 || fell) {
       double *Ld = (double *)Lraw;
+      const unsigned long long t64 = wall_clock64();
       info = wg_potrf<double, NBD, NT>(Ld, ld, n, (double *)(lds + lay.panel), S,
                                        (double *)(lds + lay.z), (double *)Linv);
+      if (a.mode == 0) tk[2] = wall_clock64();
+      else tk[2] += wall_clock64() - t64;  // the fallback factorisation counts as factorisation
       set_identity_rhs(X, n, S, NT);
       if (info == 0) wg_potrs<double, NBD, NT>(Ld, ld, n, (const double *)Linv, X, S, tmp);
       rmax = wg_residual<NT>(A64, ld, n, X, R, S, red);
@@ -660,7 +669,11 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
       s += X[2 * S + 3 * i + 2] / 3.0;
       a.alpha[(long)f * a.natoms + o + i] = s;
     }
+    tk[3] = wall_clock64();
     if (tid == 0) {
+      atomicAdd(&a.phase[0], tk[1] - tk[0]);
+      atomicAdd(&a.phase[1], tk[2] - tk[1]);
+      atomicAdd(&a.phase[2], tk[3] - tk[2]);
       if (info) atomicMax(&a.stat[4 * mol + 0], info);
       atomicMax(&a.stat[4 * mol + 2], iters);
       if (fell) atomicAdd(&a.stat[4 * mol + 3], 1);
@@ -675,6 +688,8 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     __syncthreads();
     if (!ibuf[1]) continue;
     __threadfence();
+    unsigned long long te[4];
+    te[0] = wall_clock64();
 
     // ---- coupled-QHO energy of molecule mol ------------------------------
     // aw <- alpha_eff, sg <- omega, rv <- R_eff (UTILS.f90:302-313, 737-739)
@@ -723,7 +738,9 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     __syncthreads();
     double *pw = (double *)(lds + lay.panel);
     double *dv = R + S, *ev = R + 2 * S;
+    te[1] = wall_clock64();
     wg_sytrd<NT, 4>(A64, ld, n, dv, ev, X, X + S, X + 2 * S, pw, R, S, red);
+    te[2] = wall_clock64();
     double *e2 = pw, *lam = pw + S;
     wg_tri_eigvals<NT>(n, dv, ev, e2, lam, red);
     double s = 0.0, c = 0.0, neg = 0.0;
@@ -744,6 +761,10 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     const double esum = wg_sum<NT>(en, red);
     const double nneg = wg_sum<NT>(neg, red);
     if (tid == 0) {
+      te[3] = wall_clock64();
+      atomicAdd(&a.phase[3], te[1] - te[0]);
+      atomicAdd(&a.phase[4], te[2] - te[1]);
+      atomicAdd(&a.phase[5], te[3] - te[2]);
       a.energy[mol] = 0.5 * ssum - 0.5 * esum;
       a.c6_mol[mol] = kThreeOverPi * c6m;
       a.stat[4 * mol + 1] = (int)nneg;
@@ -784,25 +805,17 @@ size_t batch_slot_bytes(int nmax) {
 }
 
 void batch_prepare(const BatchConfig &c, int nmax) {
-  if (c.nb == 16 && c.nt == 256)
+  if (c.nb == 16)
     prepare_one<16, 256>(nmax);
-  else if (c.nb == 16 && c.nt == 512)
-    prepare_one<16, 512>(nmax);
-  else if (c.nb == 32 && c.nt == 256)
-    prepare_one<32, 256>(nmax);
   else
-    prepare_one<32, 512>(nmax);
+    prepare_one<32, 256>(nmax);
 }
 
 void batch_launch(hipStream_t s, const BatchConfig &c, const BatchArgs &a, int grid) {
-  if (c.nb == 16 && c.nt == 256)
+  if (c.nb == 16)
     launch_one<16, 256>(s, a, grid);
-  else if (c.nb == 16 && c.nt == 512)
-    launch_one<16, 512>(s, a, grid);
-  else if (c.nb == 32 && c.nt == 256)
-    launch_one<32, 256>(s, a, grid);
   else
-    launch_one<32, 512>(s, a, grid);
+    launch_one<32, 256>(s, a, grid);
 }
 
 void batch_zero(hipStream_t s, int *p, int n) {

@@ -184,7 +184,7 @@ struct mbd_handle {
   struct Batch {
     int nmol = 0, natoms = 0, nmax = 0, ncu = 0;
     double beta = 1.0, rc_scs = 0.0, rc_cfdm = 0.0;  // bohr
-    DevBuf<int> off, order, counter, stat;
+    DevBuf<int> off, order, counter, stat, phase;
     DevBuf<double> xyz, hirsh, a0, c60, r0, alpha, c6, energy, c6_mol, resid;
     DevBuf<char> work;
     int cfg = -1;  // tuned candidate, -1 = default
@@ -230,6 +230,7 @@ void batch_release(mbd_handle *h) {
   b.order.release();
   b.counter.release();
   b.stat.release();
+  b.phase.release();
   b.work.release();
   b.nmol = 0;
 }
@@ -1312,7 +1313,7 @@ extern "C" int mbd_debug_copy(int device, long n_bytes, double *ms) {
 namespace {
 
 int batch_default_config(int nmax) {
-  const int pref[] = {4, 0, 6, 2};  // {32,256,1}, {16,256,1}, {32,512,1}, {16,512,1}
+  const int pref[] = {2, 0};  // {32,256,1}, {16,256,1}
   for (int c : pref)
     if (batch_lds_bytes(kBatchConfigs[c], nmax)) return c;
   return -1;
@@ -1348,6 +1349,7 @@ BatchArgs batch_args(mbd_handle *h, int mode) {
   a.stat = b.stat.p;
   a.resid = b.resid.p;
   a.counter = b.counter.p;
+  a.phase = (unsigned long long *)b.phase.p;
   return a;
 }
 
@@ -1358,6 +1360,7 @@ void batch_enqueue(mbd_handle *h, int mode, int cfg) {
   batch_zero(h->stream, b.counter.p, 1 + b.nmol);
   batch_zero(h->stream, b.stat.p, 4 * b.nmol);
   batch_zero(h->stream, (int *)b.resid.p, 2 * b.nmol);
+  batch_zero(h->stream, b.phase.p, 2 * kBatchPhases);
   batch_launch(h->stream, c, batch_args(h, mode), b.ncu * c.per_cu);
 }
 
@@ -1444,6 +1447,7 @@ int mbd_batch_set(mbd_handle *h, int nmol, const int *offsets, const double *xyz
     b.resid.ensure(nmol);
     b.counter.ensure(1 + nmol);
     b.stat.ensure(4 * nmol);
+    b.phase.ensure(2 * kBatchPhases);
     b.work.ensure(2 * (size_t)b.ncu * batch_slot_bytes(nmax));
     MBD_HIP(hipStreamSynchronize(h->stream));
     if (b.cfg >= 0 && !batch_lds_bytes(kBatchConfigs[b.cfg], nmax)) b.cfg = -1;
@@ -1562,6 +1566,11 @@ int mbd_batch_run(mbd_handle *h, int mode, double *energy, double *alpha_eff, do
         d->max_residual = std::max(d->max_residual, res[m]);
       }
       d->t_ms = ms;
+      unsigned long long ph[kBatchPhases];
+      MBD_HIP(hipMemcpy(ph, b.phase.p, sizeof ph, hipMemcpyDeviceToHost));
+      int khz = 0;
+      MBD_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device));
+      for (int k = 0; k < kBatchPhases; ++k) d->phase_ms[k] = khz > 0 ? (double)ph[k] / khz : 0.0;
     }
   } catch (const HipError &e) {
     return fail(h, e);

@@ -220,7 +220,7 @@ class MbdEngine:
 
     def batch_tune(self, mode: str = "mixed", reps: int = 3):
         """Times every eligible launch shape; returns (best index, times in ms)."""
-        t = np.zeros(8)
+        t = np.zeros(batch_num_configs())
         best = ctypes.c_int()
         self._check(self._L.mbd_batch_tune(self._h, MODES[mode], reps, dptr(t), ctypes.byref(best)),
                     "mbd_batch_tune")
@@ -247,7 +247,9 @@ class MbdEngine:
             rc = self._L.mbd_batch_run(self._h, MODES[mode], dptr(e), None, None, None, None, None,
                                        ctypes.byref(d))
         self._check(rc, "mbd_batch_run")
-        out["diag"] = {f: getattr(d, f) for f, _ in MbdBatchDiag._fields_}
+        out["diag"] = {f: getattr(d, f) for f, _ in MbdBatchDiag._fields_ if f != "phase_ms"}
+        out["diag"]["phase_ms"] = dict(zip(("relay_build", "factor", "solve_refine", "qho_build",
+                                            "tridiag", "bisect_energy"), list(d.phase_ms)))
         out["offsets"] = off
         return out
 
@@ -352,6 +354,13 @@ def device_copy(n_bytes: int, device: int = 0) -> float:
     if rc != 0:
         raise MbdError(f"mbd_debug_copy failed ({rc})")
     return ms.value
+
+
+def batch_num_configs() -> int:
+    n = 0
+    while _lib.lib().mbd_batch_config(n, None, None, None) == 0:
+        n += 1
+    return n
 
 
 def batch_config(index: int):

@@ -96,7 +96,7 @@ def test_every_launch_shape_matches_default():
         eng.batch_set(mols, st)
         ref = eng.batch_run("fp64")
         worst = {}
-        for c in range(8):
+        for c in range(engine.batch_num_configs()):
             cfg = engine.batch_config(c)
             print(f"shape {c} {cfg}", flush=True)
             eng.batch_set_config(c)
