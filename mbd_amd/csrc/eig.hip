@@ -125,13 +125,12 @@ __device__ __forceinline__ void trd_colupd_body(
     sv[lane] = (r >= j + 2 && r < n) ? val : 0.0;
   }
   __syncthreads();
-  // partials q < i: W^T a, i <= q < 2i: V^T a, q = 2i: |a|^2; each wave
-  // takes 16 quantities at a time so their loads are in flight together
+  // partials q < i: W^T a, i <= q < 2i: V^T a, q = 2i: |a|^2; one wave per q
   const double x = sv[lane];
-  for (int q0 = part * 16; q0 <= 2 * i; q0 += 64) {
-    double acc[16];
+  for (int q0 = part * 4; q0 <= 2 * i; q0 += 16) {
+    double acc[4];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < 4; ++u) {
       const int q = q0 + u;
       acc[u] = 0.0;
       if (q == 2 * i) {
@@ -142,10 +141,10 @@ __device__ __forceinline__ void trd_colupd_body(
       }
     }
 #pragma unroll
-    for (int u = 0; u < 16; ++u) acc[u] = wave_sum(acc[u]);
+    for (int u = 0; u < 4; ++u) acc[u] = wave_sum(acc[u]);
     if (lane == 0) {
 #pragma unroll
-      for (int u = 0; u < 16; ++u)
+      for (int u = 0; u < 4; ++u)
         if (q0 + u <= 2 * i) pa[(long)(q0 + u) * pald + bx] = acc[u];
     }
   }
@@ -222,9 +221,10 @@ __device__ __forceinline__ void trd_symv_body(int n, const double *__restrict__ 
   const double v0 = vI[lane], v1 = vI[lane + 64];
   double ra0 = 0.0, ra1 = 0.0;
   constexpr int CPW = TS / 4;  // columns per wave
-  constexpr int G = 8;         // columns whose loads are issued together
+  constexpr int G = 8;         // columns per group: loads in flight together
+#pragma unroll 1
   for (int g = 0; g < CPW; g += G) {
-    double a0[G], a1[G];
+    double a0[G], a1[G], cs[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       const int c = C0 + wave * CPW + g + u;
@@ -236,7 +236,6 @@ __device__ __forceinline__ void trd_symv_body(int n, const double *__restrict__ 
         if (r1 < n && (!diag || r1 >= c)) a1[u] = col[r1];
       }
     }
-    double cs[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       const int cl = wave * CPW + g + u;
@@ -249,12 +248,28 @@ __device__ __forceinline__ void trd_symv_body(int n, const double *__restrict__ 
       const double b1 = (diag && r1 == c) ? 0.0 : a1[u];
       cs[u] = b0 * v0 + b1 * v1;
     }
+    // Reduce the group's 8 column dots across the wave by recursive halving:
+    // at mask m each lane keeps the half of its remaining columns selected by
+    // lane bit m and adds the partner's copy of that half; lane l then holds
+    // column (l >> 3) over 8 lanes, and three exchanges (masks 4, 2, 1)
+    // complete the sum.
 #pragma unroll
-    for (int u = 0; u < G; ++u) cs[u] = wave_sum(cs[u]);
-    if (lane == 0) {
+    for (int step = 0; step < 3; ++step) {
+      const int h = G >> (step + 1), m = 32 >> step;
+      const bool up = (lane & m) != 0;
 #pragma unroll
-      for (int u = 0; u < G; ++u) colp[wave * CPW + g + u] = cs[u];
+      for (int k = 0; k < G / 2; ++k) {
+        if (k < h) {
+          const double send = up ? cs[k] : cs[k + h];
+          const double keep = up ? cs[k + h] : cs[k];
+          cs[k] = keep + __shfl_xor(send, m, 64);
+        }
+      }
     }
+    double ctot = cs[0] + __shfl_xor(cs[0], 4, 64);
+    ctot += __shfl_xor(ctot, 2, 64);
+    ctot += __shfl_xor(ctot, 1, 64);
+    if ((lane & 7) == 0) colp[wave * CPW + g + (lane >> 3)] = ctot;
   }
   rowp[wave][lane] = ra0;
   rowp[wave][lane + 64] = ra1;
@@ -328,15 +343,14 @@ __device__ __forceinline__ void trd_wcol_body(
   if (c < n) {
     const int Ib = (c - o) / TS, cl = (c - o) % TS;
     const double *py = Py + (long)Ib * nbt * TS + cl;
-    double ya[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) ya[u] = 0.0;
-    for (int Jb = part; Jb < nbt; Jb += 32) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (Jb + 4 * u < nbt) ya[u] += py[(long)(Jb + 4 * u) * TS];
+    double y = 0.0, y2 = 0.0;
+    int Jb = part;
+    for (; Jb + 4 < nbt; Jb += 8) {
+      y += py[(long)Jb * TS];
+      y2 += py[(long)(Jb + 4) * TS];
     }
-    const double y = ((ya[0] + ya[1]) + (ya[2] + ya[3])) + ((ya[4] + ya[5]) + (ya[6] + ya[7]));
+    if (Jb < nbt) y += py[(long)Jb * TS];
+    y += y2;
     double corr = 0.0;
 #pragma unroll 4
     for (int p = part; p < i; p += 4) {
