@@ -177,7 +177,7 @@ struct mbd_handle {
   DevBuf<double> img_net;  // SCS images that can reach within the descriptor cutoff
   int nimg_net = 1, center_net = 0;
   // device: MBD
-  DevBuf<double> C, om_at, reff, lam, e2, bounds, E, VW, tau, dd, ee, alph, pa, pb, py, qred;
+  DevBuf<double> C, om_at, reff, lam, e2, bounds, E, VW, tau, dd, ee, alph, pa, pb, py, qred, ps;
   DevBuf<double> ext_alpha, ext_c6;
   DevBuf<int> nneg;
   // batched molecules (one persistent launch, replayed from a hipGraph)
@@ -456,13 +456,15 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
   h->ee.ensure(nm);
   h->alph.ensure(kTrdNB);
   wk.pald = (int)(nm / 64 + 2);
-  h->pa.ensure((size_t)wk.pald * (2 * kTrdNB + 1));
+  h->pa.ensure((size_t)wk.pald * kTrdQuant);
   h->pb.ensure((size_t)wk.pald);
   const size_t nbt = (size_t)ceil_div(nm, kTrdTile);
   h->py.ensure(nbt * nbt * kTrdTile);
   wk.py = h->py.p;
-  h->qred.ensure(2 * kTrdNB + 1);
+  h->qred.ensure(kTrdQred);
   wk.qred = h->qred.p;
+  h->ps.ensure(nbt + 1);
+  wk.ps = h->ps.p;
   wk.VW = h->VW.p;
   wk.ldv = nm;
   wk.tau = h->tau.p;
@@ -477,7 +479,7 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
   } else {
     const std::vector<const void *> key = {(const void *)(intptr_t)nm, h->C.p, h->VW.p,
                                            h->tau.p, h->dd.p, h->ee.p, h->alph.p, h->pa.p,
-                                           h->pb.p, h->py.p, h->qred.p};
+                                           h->pb.p, h->py.p, h->qred.p, h->ps.p};
     if (!h->trd_exec || h->trd_key != key) {
       if (h->trd_exec) (void)hipGraphExecDestroy(h->trd_exec);
       h->trd_exec = nullptr;
@@ -590,7 +592,7 @@ int mbd_destroy(mbd_handle *h) {
                              &h->P, &h->eig3, &h->c6mol, &h->c6, &h->R, &h->D, &h->rnorm,
                              &h->C, &h->om_at, &h->reff, &h->lam, &h->e2, &h->bounds,
                              &h->E, &h->VW, &h->tau, &h->dd, &h->ee, &h->alph, &h->pa,
-                             &h->pb, &h->py, &h->qred, &h->pMinv, &h->pR, &h->pZ, &h->pP, &h->pQ, &h->pscal, &h->ppart, &h->alpha_pred, &h->img_net, &h->ext_alpha, &h->ext_c6};
+                             &h->pb, &h->py, &h->qred, &h->ps, &h->pMinv, &h->pR, &h->pZ, &h->pP, &h->pQ, &h->pscal, &h->ppart, &h->alpha_pred, &h->img_net, &h->ext_alpha, &h->ext_c6};
   for (auto *b : dbufs) b->release();
   h->A32.release();
   h->L32.release();
@@ -1206,11 +1208,12 @@ int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w) {
     wk.e = alloc(n);
     wk.alpha = alloc(kTrdNB);
     wk.pald = n / 64 + 2;
-    wk.pa = alloc((size_t)wk.pald * (2 * kTrdNB + 1));
+    wk.pa = alloc((size_t)wk.pald * kTrdQuant);
     wk.pb = alloc((size_t)wk.pald);
     const size_t nbt = (size_t)ceil_div(n, kTrdTile);
     wk.py = alloc(nbt * nbt * kTrdTile);
-    wk.qred = alloc(2 * kTrdNB + 1);
+    wk.qred = alloc(kTrdQred);
+    wk.ps = alloc(nbt + 1);
     tridiagonalize(nullptr, n, dA, n, wk);
     double *e2 = alloc(n), *bounds = alloc(4), *lam = alloc(n);
     tridiag_eigvals(nullptr, n, wk.d, wk.e, e2, bounds, lam);

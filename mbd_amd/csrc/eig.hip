@@ -3,15 +3,12 @@
 //
 // Tridiagonalisation: blocked Householder on the lower triangle (LAPACK's
 // sytrd/latrd organisation), reading and updating only the lower triangle.
-// Per column j three kernels:
-//   colupd: apply the panel's earlier reflectors to column j, finalise the
-//           previous W column's scalar, and emit partial sums of |a|^2,
-//           W^T a and V^T a (wave-parallel reductions);
-//   symv:   y = A22 v over 128 x 128 lower tiles, the HBM-bound step; each
-//           tile emits its row-dot and column-dot contributions into a
-//           per-tile partial buffer (deterministic, no atomics);
-//   wcol:   reflector scalars, y from the partials, the W and V columns, and
-//           partial dots for the next scalar.
+// Per column j two kernels:
+//   symv: z = A22 x over 128 x 128 lower tiles, the HBM-bound step; each
+//         tile emits its row-dot and column-dot contributions into a
+//         per-tile partial buffer (deterministic, no atomics);
+//   wcol: reflector scalars, y = A22 v from the partials, the W and V
+//         columns, and the next column's update with its partial dots.
 // W columns are kept un-finalised (W_final = W + alpha_p V) so no kernel
 // rewrites data another workgroup of the same launch reads.  After each panel
 // of NBT columns one lower-triangle MFMA SYR2K (GEMM with K = 2 NBT on the
@@ -29,7 +26,6 @@ namespace mbd {
 
 namespace {
 constexpr int NBT = kTrdNB;
-constexpr int PA_STRIDE = 2 * NBT + 1;
 constexpr int TS = kTrdTile;  // symv tile edge
 
 __device__ inline double wave_sum(double v) {
@@ -44,15 +40,6 @@ __device__ inline void tri_index(int idx, int &ti, int &tj) {
   while (t * (t + 1) / 2 > idx) --t;
   ti = t;
   tj = idx - t * (t + 1) / 2;
-}
-
-// Sum of the npa workgroup partials of quantity q (layout pa[q * pald + w]),
-// reduced by one wave; every lane returns the total.
-__device__ inline double wave_reduce_partials(const double *pa, int pald, int npa, int q,
-                                              int lane) {
-  double acc = 0.0;
-  for (int w = lane; w < npa; w += 64) acc += pa[(long)q * pald + w];
-  return wave_sum(acc);
 }
 
 // Reflector scalars from |a|^2 and the head element: beta, tau and the scale
@@ -71,152 +58,114 @@ __device__ inline void reflector(double xn2, double alp, double &tau, double &be
 }
 }  // namespace
 
-// 64 rows per workgroup, 4 threads per row (the p-sums split by p mod 4).
-__device__ __forceinline__ void trd_colupd_body(
-    int n, double *__restrict__ A, long lda, double *__restrict__ VW, long ldv,
-    int j0, int j, const double *__restrict__ tau, double *__restrict__ alpha_pan,
-    const double *__restrict__ pb_part, int npb, double *__restrict__ pa, int pald,
-    double *__restrict__ d, int bx) {
-  __syncthreads();
-  const int i = j - j0;
-  __shared__ double al[NBT], vj[NBT], wj[NBT];
-  __shared__ double red[4][64];
-  __shared__ double sv[64];
-  const int t = threadIdx.x, lane = t & 63, part = t >> 6;
-  const double *V = VW;
-  const double *W = VW + (long)NBT * ldv;
-  if (i > 0 && part == 0) {
-    double acc = 0.0;
-    for (int w = lane; w < npb; w += 64) acc += pb_part[w];
-    acc = wave_sum(acc);
-    if (lane == 0) {
-      const double a = -0.5 * tau[j - 1] * acc;
-      al[i - 1] = a;
-      if (bx == 0) alpha_pan[i - 1] = a;
-    }
-  }
-  for (int p = t; p < i - 1; p += blockDim.x) al[p] = alpha_pan[p];
-  __syncthreads();
-  for (int p = t; p < i; p += blockDim.x) {
-    const double v = V[j + (long)p * ldv];
-    vj[p] = v;
-    wj[p] = W[j + (long)p * ldv] + al[p] * v;
-  }
-  __syncthreads();
-  const int r = j + bx * 64 + lane;
-  double s = 0.0;
-  if (r < n) {
-#pragma unroll 4
-    for (int p = part; p < i; p += 4) {
-      const double vr = V[r + (long)p * ldv];
-      const double wr = W[r + (long)p * ldv] + al[p] * vr;
-      s += vr * wj[p] + wr * vj[p];
-    }
-  }
-  red[part][lane] = s;
-  __syncthreads();
-  if (part == 0) {
-    double val = 0.0;
-    if (r < n) {
-      val = A[r + (long)j * lda] - ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
-      A[r + (long)j * lda] = val;
-      if (r == j) d[j] = val;
-    }
-    sv[lane] = (r >= j + 2 && r < n) ? val : 0.0;
-  }
-  __syncthreads();
-  // partials q < i: W^T a, i <= q < 2i: V^T a, q = 2i: |a|^2; one wave per q
-  const double x = sv[lane];
-  for (int q0 = part * 4; q0 <= 2 * i; q0 += 16) {
-    double acc[4];
+// ---- Two launches per Householder column --------------------------------
+// Step i of a panel (column j = j0 + i, trailing block from o = j + 1).
+// Column j arrives in A as b'': the W column kernel of step i - 1 applied to
+// it every reflector of the panel except the part 2 alpha_{i-1} V[:, i-1] of
+// the newest one, whose scalar alpha_{i-1} = -tau_{j-1} w^T v / 2 needs a sum
+// over the whole grid.  So the updated column is
+//   u = b'' - kappa V[:, i-1],  kappa = 2 alpha_{i-1}   (u = A[:, j] at i = 0),
+// x = u below the head (rows >= o + 1) and the reflector is v = e_o + sc x.
+// k_trd_symv forms kappa, z = A22 x (scale-free), the squared-norm partials
+// of x and the reduced partial dots; k_trd_wcol forms sc, tau, beta, then
+// y = A22 v = sc z + A22 e_o, the W / V columns, and column j + 1's b'' with
+// the partial dots W^T b'', V^T b'', W^T v, V^T v from which the next step's
+// dots W^T x', V^T x' follow by linearity (|x'|^2 is summed directly).
+// Row o of b'' goes to a separate slot (bdiag): every workgroup of the W
+// column kernel still reads A[o][o].
+
+// Reduce the 8 per-lane values q[0..7] over the wave by recursive halving;
+// afterwards lane l holds the total of value (l >> 3).
+__device__ inline double halve8(double *q, int lane) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int q = q0 + u;
-      acc[u] = 0.0;
-      if (q == 2 * i) {
-        acc[u] = x * x;
-      } else if (q < 2 * i && r < n) {
-        const double *col = (q < i) ? (W + (long)q * ldv) : (V + (long)(q - i) * ldv);
-        acc[u] = col[r] * x;
+  for (int step = 0; step < 3; ++step) {
+    const int h = 4 >> step, m = 32 >> step;
+    const bool up = (lane & m) != 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k < h) {
+        const double send = up ? q[k] : q[k + h];
+        const double keep = up ? q[k + h] : q[k];
+        q[k] = keep + __shfl_xor(send, m, 64);
       }
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) acc[u] = wave_sum(acc[u]);
-    if (lane == 0) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (q0 + u <= 2 * i) pa[(long)(q0 + u) * pald + bx] = acc[u];
-    }
   }
+  double t = q[0] + __shfl_xor(q[0], 4, 64);
+  t += __shfl_xor(t, 2, 64);
+  t += __shfl_xor(t, 1, 64);
+  return t;
 }
 
-__global__ __launch_bounds__(256) void k_trd_colupd(
-    int n, double *__restrict__ A, long lda, double *__restrict__ VW, long ldv,
+// z = A22 x on the lower triangle, tile (I, J), I >= J: the tile's row dots
+// go to Py[I][J] and its column dots to Py[J][I] (both to Py[I][I] on the
+// diagonal); diagonal tiles also write |x|^2 over their rows to ps[I].
+__global__ __launch_bounds__(256) void k_trd_symv(
+    int n, const double *__restrict__ A, long lda, const double *__restrict__ VW, long ldv,
     int j0, int j, const double *__restrict__ tau, double *__restrict__ alpha_pan,
-    const double *__restrict__ pb_part, int npb, double *__restrict__ pa, int pald,
-    double *__restrict__ d) {
-  trd_colupd_body(n, A, lda, VW, ldv, j0, j, tau, alpha_pan, pb_part, npb, pa, pald, d, blockIdx.x);
-}
-
-// y = A22 v on the lower triangle, tile (I, J), I >= J, of the trailing
-// matrix starting at row/column o = j + 1.  Writes the tile's row dots to
-// Py[I][J] and its column dots to Py[J][I] (on the diagonal, both to Py[I][I]).
-__device__ __forceinline__ void trd_symv_body(int n, const double *__restrict__ A,
-                                                  long lda, int j0, int j,
-                                                  const double *__restrict__ pa, int pald,
-                                                  int npa, int nbt,
-                                                  double *__restrict__ Py,
-                                                  double *__restrict__ qred, int bx, int nb) {
-  __syncthreads();
+    const double *__restrict__ pb_part, int npb, const double *__restrict__ pa, int pald,
+    int npa, int nbt, double *__restrict__ Py, double *__restrict__ qred,
+    double *__restrict__ ps, const double *__restrict__ bdiag, double *__restrict__ d) {
   __shared__ double vI[TS], vJ[TS];
   __shared__ double rowp[4][TS];
   __shared__ double colp[TS];
-  __shared__ double sc_s;
+  __shared__ double kap_s;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int i = j - j0;
   int I, J;
-  tri_index(bx, I, J);
+  tri_index(blockIdx.x, I, J);
   const int o = j + 1;
   const double *colj = A + (long)j * lda;
-  {
-    // reduce every colupd partial once for k_trd_wcol: qred[0..2i], eight
-    // quantities per wave, spread over the workgroups of the launch
-    for (int q0 = (bx * 4 + wave) * 8; q0 <= 2 * i; q0 += nb * 32) {
-      double acc[8];
+  const double *Vp = VW + (long)(i > 0 ? i - 1 : 0) * ldv;  // newest reflector
+  // the W column kernel's 4i partial dots, eight quantities per wave,
+  // spread over the workgroups of the launch
+  const int nq = 4 * i;
+  for (int q0 = (blockIdx.x * 4 + wave) * 8; q0 < nq; q0 += gridDim.x * 32) {
+    double acc[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc[u] = 0.0;
-      for (int w = lane; w < npa; w += 64) {
+    for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+    for (int w = lane; w < npa; w += 64) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-          if (q0 + u <= 2 * i) acc[u] += pa[(long)(q0 + u) * pald + w];
-      }
+      for (int u = 0; u < 8; ++u)
+        if (q0 + u < nq) acc[u] += pa[(long)(q0 + u) * pald + w];
+    }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc[u] = wave_sum(acc[u]);
-      if (lane == 0) {
+    for (int u = 0; u < 8; ++u) acc[u] = wave_sum(acc[u]);
+    if (lane == 0) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-          if (q0 + u <= 2 * i) qred[q0 + u] = acc[u];
-      }
+      for (int u = 0; u < 8; ++u)
+        if (q0 + u < nq) qred[q0 + u] = acc[u];
     }
   }
   if (wave == 0) {
-    const double xn2 = wave_reduce_partials(pa, pald, npa, 2 * i, lane);
+    double kap = 0.0;
+    if (i > 0) {
+      double s = 0.0;
+      for (int w = lane; w < npb; w += 64) s += pb_part[w];
+      s = wave_sum(s);
+      const double a = -0.5 * tau[j - 1] * s;
+      kap = 2.0 * a;
+      if (lane == 0 && blockIdx.x == 0) alpha_pan[i - 1] = a;
+    }
     if (lane == 0) {
-      double tv, beta, sc;
-      reflector(xn2, colj[o], tv, beta, sc);
-      sc_s = sc;
+      kap_s = kap;
+      if (blockIdx.x == 0) d[j] = (i > 0) ? *bdiag - kap : colj[j];  // V[j][i-1] = 1
     }
   }
   __syncthreads();
-  const double sc = sc_s;
+  const double kap = kap_s;
   const int R0 = o + I * TS, C0 = o + J * TS;
   for (int k = t; k < TS; k += blockDim.x) {
     const int r = R0 + k, c = C0 + k;
-    vI[k] = (r < n) ? ((r == o) ? 1.0 : sc * colj[r]) : 0.0;
-    vJ[k] = (c < n) ? ((c == o) ? 1.0 : sc * colj[c]) : 0.0;
+    vI[k] = (r < n && r > o) ? (i > 0 ? colj[r] - kap * Vp[r] : colj[r]) : 0.0;
+    vJ[k] = (c < n && c > o) ? (i > 0 ? colj[c] - kap * Vp[c] : colj[c]) : 0.0;
   }
   __syncthreads();
   const bool diag = (I == J);
+  if (diag && wave == 0) {
+    double s2 = vI[lane] * vI[lane] + vI[lane + 64] * vI[lane + 64];
+    s2 = wave_sum(s2);
+    if (lane == 0) ps[I] = s2;
+  }
   const int r0 = R0 + lane, r1 = R0 + lane + 64;
   const double v0 = vI[lane], v1 = vI[lane + 64];
   double ra0 = 0.0, ra1 = 0.0;
@@ -248,27 +197,7 @@ __device__ __forceinline__ void trd_symv_body(int n, const double *__restrict__ 
       const double b1 = (diag && r1 == c) ? 0.0 : a1[u];
       cs[u] = b0 * v0 + b1 * v1;
     }
-    // Reduce the group's 8 column dots across the wave by recursive halving:
-    // at mask m each lane keeps the half of its remaining columns selected by
-    // lane bit m and adds the partner's copy of that half; lane l then holds
-    // column (l >> 3) over 8 lanes, and three exchanges (masks 4, 2, 1)
-    // complete the sum.
-#pragma unroll
-    for (int step = 0; step < 3; ++step) {
-      const int h = G >> (step + 1), m = 32 >> step;
-      const bool up = (lane & m) != 0;
-#pragma unroll
-      for (int k = 0; k < G / 2; ++k) {
-        if (k < h) {
-          const double send = up ? cs[k] : cs[k + h];
-          const double keep = up ? cs[k + h] : cs[k];
-          cs[k] = keep + __shfl_xor(send, m, 64);
-        }
-      }
-    }
-    double ctot = cs[0] + __shfl_xor(cs[0], 4, 64);
-    ctot += __shfl_xor(ctot, 2, 64);
-    ctot += __shfl_xor(ctot, 1, 64);
+    const double ctot = halve8(cs, lane);
     if ((lane & 7) == 0) colp[wave * CPW + g + (lane >> 3)] = ctot;
   }
   rowp[wave][lane] = ra0;
@@ -287,101 +216,146 @@ __device__ __forceinline__ void trd_symv_body(int n, const double *__restrict__ 
   }
 }
 
-__global__ __launch_bounds__(256) void k_trd_symv(int n, const double *__restrict__ A,
-                                                  long lda, int j0, int j,
-                                                  const double *__restrict__ pa, int pald,
-                                                  int npa, int nbt,
-                                                  double *__restrict__ Py,
-                                                  double *__restrict__ qred) {
-  trd_symv_body(n, A, lda, j0, j, pa, pald, npa, nbt, Py, qred, blockIdx.x, gridDim.x);
-}
-
-// W and V columns for rows o..n-1 from the symv partials; 64 rows per
-// workgroup, 4 threads per row.
-__device__ __forceinline__ void trd_wcol_body(
-    int n, const double *__restrict__ A, long lda, double *__restrict__ VW, long ldv,
-    int j0, int j, double *__restrict__ tau, double *__restrict__ e,
-    const double *__restrict__ alpha_pan, const double *__restrict__ qred, int nbt,
-    const double *__restrict__ Py, double *__restrict__ pb_part, int bx) {
-  __syncthreads();
+// W and V columns of step i for rows o..n-1, and (next != 0) column j + 1's
+// b'' with the next step's partial dots; 64 rows per workgroup, 4 threads
+// per row (the p-sums split by p mod 4).
+__global__ __launch_bounds__(256) void k_trd_wcol(
+    int n, double *__restrict__ A, long lda, double *__restrict__ VW, long ldv, int j0, int j,
+    double *__restrict__ tau, double *__restrict__ e, const double *__restrict__ alpha_pan,
+    const double *__restrict__ qred, int nbt, const double *__restrict__ Py,
+    const double *__restrict__ ps, double *__restrict__ pb_part, double *__restrict__ pa,
+    int pald, double *__restrict__ bdiag, int next) {
   const int i = j - j0;
-  __shared__ double q_s[PA_STRIDE];
-  __shared__ double al[NBT], t1[NBT], t2[NBT];
-  __shared__ double red[4][64];
-  __shared__ double sc_s, tau_s;
+  __shared__ double al[NBT], t1[NBT], t2[NBT], vo[NBT], wfo[NBT];
+  __shared__ double red[4][64], red2[4][64];
+  __shared__ double sb[64], sv[64];
+  __shared__ double sc_s, tau_s, kap_s, wo_s;
   const int t = threadIdx.x, lane = t & 63, part = t >> 6;
   double *V = VW;
   double *W = VW + (long)NBT * ldv;
   const int o = j + 1;
   const double *colj = A + (long)j * lda;
-  for (int q = t; q <= 2 * i; q += blockDim.x) q_s[q] = qred[q];
+  double *colo = A + (long)o * lda;
+  const double *Vp = V + (long)(i > 0 ? i - 1 : 0) * ldv;
   for (int p = t; p < i; p += blockDim.x) al[p] = alpha_pan[p];
   __syncthreads();
-  if (t == 0) {
-    double tv, beta, sc;
-    reflector(q_s[2 * i], colj[o], tv, beta, sc);
-    tau_s = tv;
-    sc_s = sc;
-    if (bx == 0) {
-      tau[j] = tv;
-      e[j] = beta;
+  if (part == 0) {
+    double sg = 0.0;
+    for (int b = lane; b < nbt; b += 64) sg += ps[b];
+    sg = wave_sum(sg);
+    if (lane == 0) {
+      const double kap = (i > 0) ? 2.0 * al[i - 1] : 0.0;
+      const double head = (i > 0) ? colj[o] - kap * Vp[o] : colj[o];
+      double tv, beta, sc;
+      reflector(sg, head, tv, beta, sc);
+      tau_s = tv;
+      sc_s = sc;
+      kap_s = kap;
+      if (blockIdx.x == 0) {
+        tau[j] = tv;
+        e[j] = beta;
+      }
     }
   }
   __syncthreads();
-  const double sc = sc_s, tv = tau_s;
+  const double sc = sc_s, tv = tau_s, kap = kap_s;
   for (int p = t; p < i; p += blockDim.x) {
     const double vh = V[o + (long)p * ldv];
-    const double wh = W[o + (long)p * ldv] + al[p] * vh;
-    const double S2 = q_s[i + p];
-    const double S1 = q_s[p] + al[p] * S2;
-    t1[p] = wh + sc * S1;
+    const double wf = W[o + (long)p * ldv] + al[p] * vh;
+    const double S2 = qred[i + p] - kap * qred[3 * i + p];
+    const double S1 = (qred[p] - kap * qred[2 * i + p]) + al[p] * S2;
+    t1[p] = wf + sc * S1;
     t2[p] = vh + sc * S2;
+    vo[p] = vh;
+    wfo[p] = wf;
   }
   __syncthreads();
-  const int c = o + bx * 64 + lane;
-  double part_sum = 0.0;
+  // w at row o, formed the same way in every workgroup (the next column's
+  // update needs it and only workgroup 0 owns row o)
+  if (part == 0) {
+    double zs = 0.0, cs = 0.0;
+    for (int b = lane; b < nbt; b += 64) zs += Py[(long)b * TS];  // row block 0, row 0
+    for (int p = lane; p < i; p += 64) cs += vo[p] * t1[p] + wfo[p] * t2[p];
+    zs = wave_sum(zs);
+    cs = wave_sum(cs);
+    if (lane == 0) wo_s = tv * ((sc * zs + colo[o]) - cs);
+  }
+  const int c = o + blockIdx.x * 64 + lane;
+  double zc = 0.0, corr = 0.0, nx = 0.0;
   if (c < n) {
     const int Ib = (c - o) / TS, cl = (c - o) % TS;
     const double *py = Py + (long)Ib * nbt * TS + cl;
-    double y = 0.0, y2 = 0.0;
+    double y2 = 0.0;
     int Jb = part;
     for (; Jb + 4 < nbt; Jb += 8) {
-      y += py[(long)Jb * TS];
+      zc += py[(long)Jb * TS];
       y2 += py[(long)(Jb + 4) * TS];
     }
-    if (Jb < nbt) y += py[(long)Jb * TS];
-    y += y2;
-    double corr = 0.0;
+    if (Jb < nbt) zc += py[(long)Jb * TS];
+    zc += y2;
 #pragma unroll 4
     for (int p = part; p < i; p += 4) {
       const double vp = V[c + (long)p * ldv];
       const double wp = W[c + (long)p * ldv] + al[p] * vp;
       corr += vp * t1[p] + wp * t2[p];
+      nx += vp * wfo[p] + wp * vo[p];
     }
-    part_sum = y - corr;
   }
-  red[part][lane] = part_sum;
+  red[part][lane] = sc * zc - corr;
+  red2[part][lane] = nx;
   __syncthreads();
   if (part == 0) {
-    double contrib = 0.0;
+    double contrib = 0.0, bn = 0.0, vn = 0.0;
     if (c < n) {
-      const double vcc = (c == o) ? 1.0 : sc * colj[c];
-      const double wcc = tv * ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
+      const double yc = ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane])) + colo[c];
+      const double wcc = (c == o) ? wo_s : tv * yc;
+      const double xc = (i > 0) ? colj[c] - kap * Vp[c] : colj[c];
+      const double vcc = (c == o) ? 1.0 : sc * xc;
       W[c + (long)i * ldv] = wcc;
       V[c + (long)i * ldv] = vcc;
       contrib = wcc * vcc;
+      if (next) {
+        // b'' of column o: every reflector p < i, and the newest one but for
+        // its 2 alpha_i V[:, i] part (V[o][i] = 1)
+        const double b2 = colo[c] - ((red2[0][lane] + red2[1][lane]) + (red2[2][lane] + red2[3][lane])) -
+                          vcc * wo_s - wcc;
+        if (c == o)
+          *bdiag = b2;
+        else
+          colo[c] = b2;
+        if (c >= o + 2) {
+          bn = b2;
+          vn = vcc;
+        }
+      }
     }
+    sb[lane] = bn;
+    sv[lane] = vn;
     contrib = wave_sum(contrib);
-    if (lane == 0) pb_part[bx] = contrib;
+    if (lane == 0) pb_part[blockIdx.x] = contrib;
   }
-}
-
-__global__ __launch_bounds__(256) void k_trd_wcol(
-    int n, const double *__restrict__ A, long lda, double *__restrict__ VW, long ldv,
-    int j0, int j, double *__restrict__ tau, double *__restrict__ e,
-    const double *__restrict__ alpha_pan, const double *__restrict__ qred, int nbt,
-    const double *__restrict__ Py, double *__restrict__ pb_part) {
-  trd_wcol_body(n, A, lda, VW, ldv, j0, j, tau, e, alpha_pan, qred, nbt, Py, pb_part, blockIdx.x);
+  if (!next) return;
+  __syncthreads();
+  // partial dots of the next step over this workgroup's rows >= o + 2, its
+  // i' = i + 1 reflectors: [0, i') W^T b'', [i', 2i') V^T b'',
+  // [2i', 3i') W^T v_i, [3i', 4i') V^T v_i; eight quantities per wave at a time
+  const int ip = i + 1, nq = 4 * ip;
+  const double xb = sb[lane], xv = sv[lane];
+  for (int q0 = part * 8; q0 < nq; q0 += 32) {
+    double qv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int q = q0 + u;
+      qv[u] = 0.0;
+      if (q < nq && c < n) {
+        const int blk = q / ip, p = q - blk * ip;
+        const double *col = ((blk & 1) ? V : W) + (long)p * ldv;
+        qv[u] = col[c] * ((blk < 2) ? xb : xv);
+      }
+    }
+    const double tot = halve8(qv, lane);
+    if ((lane & 7) == 0 && q0 + (lane >> 3) < nq) pa[(long)(q0 + (lane >> 3)) * pald + blockIdx.x] = tot;
+  }
 }
 
 // Finalise the panel's W columns (W += alpha V), duplicate V after W so
@@ -506,29 +480,23 @@ void tridiagonalize(hipStream_t s, int n, double *A, long lda, TrdWork &wk) {
     int npb = 0;
     for (int i = 0; i < pb; ++i) {
       const int j = j0 + i;
-      const int npa = ceil_div(n - j, 64);
-      {
-        ProfScope ps(K_TRD_COLUPD, s, 8.0 * (n - j) * (2.0 * i + 1.0), 4.0 * (n - j) * i);
-        hipLaunchKernelGGL(k_trd_colupd, dim3(npa), dim3(256), 0, s, n, A, lda,
-                           wk.VW, wk.ldv, j0, j, wk.tau, wk.alpha, wk.pb, npb,
-                           wk.pa, wk.pald, wk.d);
-        MBD_LAUNCH_CHECK();
-      }
       const int m = n - j - 1;
       const int nbt = ceil_div(m, TS);
+      const int nprev = ceil_div(n - j, 64);  // W column blocks of step i - 1
       {
         // algorithmic: the symmetric product reads the lower triangle once
         ProfScope ps(K_TRD_SYMV, s, 8.0 * 0.5 * m * (m + 1.0), 2.0 * (double)m * m);
-        hipLaunchKernelGGL(k_trd_symv, dim3(nbt * (nbt + 1) / 2), dim3(256), 0, s, n,
-                           A, lda, j0, j, wk.pa, wk.pald, npa, nbt, wk.py, wk.qred);
+        hipLaunchKernelGGL(k_trd_symv, dim3(nbt * (nbt + 1) / 2), dim3(256), 0, s, n, A, lda,
+                           wk.VW, wk.ldv, j0, j, wk.tau, wk.alpha, wk.pb, npb, wk.pa, wk.pald,
+                           nprev, nbt, wk.py, wk.qred, wk.ps, wk.qred + kTrdQred - 1, wk.d);
         MBD_LAUNCH_CHECK();
       }
       const int nwc = ceil_div(m, 64);
       {
-        ProfScope ps(K_TRD_WCOL, s, 8.0 * m * (2.0 * i + nbt), 4.0 * (double)m * i);
-        hipLaunchKernelGGL(k_trd_wcol, dim3(nwc), dim3(256), 0, s, n, A, lda, wk.VW,
-                           wk.ldv, j0, j, wk.tau, wk.e, wk.alpha, wk.qred, nbt, wk.py,
-                           wk.pb);
+        ProfScope ps(K_TRD_WCOL, s, 8.0 * m * (4.0 * i + nbt), 8.0 * (double)m * i);
+        hipLaunchKernelGGL(k_trd_wcol, dim3(nwc), dim3(256), 0, s, n, A, lda, wk.VW, wk.ldv, j0,
+                           j, wk.tau, wk.e, wk.alpha, wk.qred, nbt, wk.py, wk.ps, wk.pb, wk.pa,
+                           wk.pald, wk.qred + kTrdQred - 1, (int)(i + 1 < pb));
         MBD_LAUNCH_CHECK();
       }
       npb = nwc;

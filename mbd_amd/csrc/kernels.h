@@ -43,6 +43,10 @@ void c6_sums(hipStream_t s, int N, const double *alpha, const double *P,
              const double *weight, double *eig, double *c6_mol, double *c6);
 
 // eig.hip
+// partial-dot quantities of one step (4 per reflector) and the reduced-dot
+// buffer, whose last entry holds the diagonal of the next column's update
+constexpr int kTrdQuant = 4 * kTrdNB;
+constexpr int kTrdQred = kTrdQuant + 8;
 struct TrdWork {
   double *VW;     // n x 3*kTrdNB: [V | W | V]
   long ldv;
@@ -50,11 +54,12 @@ struct TrdWork {
   double *d;      // n
   double *e;      // n
   double *alpha;  // kTrdNB
-  double *pa;     // colupd partials, (2 kTrdNB + 1) x pald, q-major
-  int pald;       // >= number of colupd workgroups, n/64 + 2
-  double *pb;     // partial dots of wcol, n/64 + 2
-  double *qred;   // reduced colupd partials, 2 kTrdNB + 1
+  double *pa;     // W-column partial dots, kTrdQuant x pald, q-major
+  int pald;       // >= number of W-column workgroups, n/64 + 2
+  double *pb;     // partial w.v dots of the W-column kernel, n/64 + 2
+  double *qred;   // reduced partial dots, kTrdQred
   double *py;     // symv tile partials, nbt^2 x kTrdTile, nbt = ceil(n/kTrdTile)
+  double *ps;     // squared-norm partials of the column, ceil(n/kTrdTile) + 1
 };
 void tridiagonalize(hipStream_t s, int n, double *A, long lda, TrdWork &wk);
 // one-time kernel attributes of tridiagonalize (call before capturing it)
