@@ -66,18 +66,19 @@ def roofline_entry(prof, cls, bound):
             / max(k["launches"], 1)}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trd_symv_c3_traffic.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trd_symv_traffic_n4096.md")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or
-    None when no summary covers it."""
+def measured_traffic_ratio(kernel):
+    """HBM bytes per launch of `kernel` over its algorithmic bytes, from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled
+    by what the counter reads for a 1 GiB copy), or None when no summary
+    covers the kernel."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
         t = line.strip().strip("`").split()
-        if len(t) == 3 and t[0] == f"traffic_bytes_per_launch[{kernel}]" and t[1] == "=":
+        if len(t) == 3 and t[0] == f"traffic_ratio[{kernel}]" and t[1] == "=":
             return float(t[2])
     return None
 
@@ -215,7 +216,12 @@ def main():
              "refine_residual": "hbm", "pcg_symv": "hbm", "chol_syrk_gemm": "fp32_tflops",
              "chol_trsm_gemm": "fp32_tflops", "trd_syr2k_gemm": "fp64_tflops"}.get(dom, "hbm")
     out["roofline"] = roofline_entry(prof_m, dom, bound)
-    out["roofline"]["traffic"] = measured_traffic(dom)
+    ratio = measured_traffic_ratio(dom)
+    if ratio is not None:
+        out["roofline"]["traffic"] = ratio * out["roofline"]["algorithmic_per_launch"]
+        out["roofline"]["traffic_basis"] = (
+            f"counter ratio {ratio} (FETCH_SIZE+WRITE_SIZE over algorithmic bytes, measured at "
+            f"n=4096) times the algorithmic bytes per launch of this run")
     others = {}
     for k in ("trd_symv", "pcg_symv", "chol_syrk_gemm"):
         if k in prof_m and prof_m[k]["launches"] and k != dom:
