@@ -214,7 +214,8 @@ def main():
     dom = max(prof_m, key=lambda k: prof_m[k]["ms"])
     bound = {"trd_symv": "hbm", "scs_build": "hbm", "cfdm_build": "hbm", "trsv": "hbm",
              "refine_residual": "hbm", "pcg_symv": "hbm", "chol_syrk_gemm": "fp32_tflops",
-             "chol_trsm_gemm": "fp32_tflops", "trd_syr2k_gemm": "fp64_tflops"}.get(dom, "hbm")
+             "chol_trsm_gemm": "fp32_tflops", "trd_syr2k_gemm": "fp64_tflops",
+             "sbr_symm": "fp64_tflops", "sbr_panel_qr": "hbm", "sbr_chase": "hbm"}.get(dom, "hbm")
     out["roofline"] = roofline_entry(prof_m, dom, bound)
     ratio = measured_traffic_ratio(dom)
     if ratio is not None:
@@ -223,9 +224,11 @@ def main():
             f"counter ratio {ratio} (FETCH_SIZE+WRITE_SIZE over algorithmic bytes, measured at "
             f"n=4096) times the algorithmic bytes per launch of this run")
     others = {}
-    for k in ("trd_symv", "pcg_symv", "chol_syrk_gemm"):
+    for k in ("trd_symv", "pcg_symv", "chol_syrk_gemm", "sbr_symm", "trd_syr2k_gemm", "sbr_chase"):
         if k in prof_m and prof_m[k]["launches"] and k != dom:
-            others[k] = roofline_entry(prof_m, k, {"chol_syrk_gemm": "fp32_tflops"}.get(k, "hbm"))
+            others[k] = roofline_entry(prof_m, k, {"chol_syrk_gemm": "fp32_tflops",
+                                                   "sbr_symm": "fp64_tflops",
+                                                   "trd_syr2k_gemm": "fp64_tflops"}.get(k, "hbm"))
     out["roofline_other"] = others
     eng.close()
     if not args.no_batch:

@@ -42,6 +42,7 @@ typedef struct mbd_handle mbd_handle;
 #define MBD_ERR_NOMEM 1003   /* device allocation failed */
 #define MBD_ERR_NONET 1004   /* MBD_MODE_LEARNED without network weights */
 #define MBD_ERR_TOOBIG 1005  /* molecule larger than the batched kernel holds */
+#define MBD_ERR_EIG 1006     /* eigen reduction abandoned (bulge-chasing wait limit) */
 
 /* Diagnostics of one call (all times in milliseconds of device time). */
 typedef struct {

@@ -195,6 +195,12 @@ struct mbd_handle {
   // n = 12288); rebuilt when the size or any buffer address changes
   hipGraphExec_t trd_exec = nullptr;
   std::vector<const void *> trd_key;
+  // two-stage eigen reduction work space; eig_flag is the bulge-chasing abort
+  // flag of the last run_energy, read back with the energy
+  DevBuf<double> sbrV, sbrT, sbrR, sbrX, sbrY, sbrMp, sbrM, sbrAB;
+  DevBuf<int> sbrCnt, sbrMcnt, sbrProg, sbrFlag;
+  int eig_used = 1;
+  int eig_flag = 0;
   // instrumentation
   bool prof_on = false;
   Profiler prof;
@@ -473,13 +479,43 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
   wk.alpha = h->alph.p;
   wk.pa = h->pa.p;
   wk.pb = h->pb.p;
+  const int method = eig_method((int)nm);
+  h->eig_used = method;
+  h->eig_flag = 0;
+  SbrWork sw{};
+  if (method == 2) {
+    const SbrSizes z = sbr_sizes((int)nm);
+    h->sbrV.ensure(z.V);
+    h->sbrT.ensure(z.TR);
+    h->sbrR.ensure(z.TR);
+    h->sbrX.ensure(z.X);
+    h->sbrY.ensure(z.Yp);
+    h->sbrMp.ensure(z.Mp);
+    h->sbrM.ensure((size_t)kSbrB * kSbrB);
+    h->sbrAB.ensure(z.AB);
+    h->sbrCnt.ensure(z.cnt);
+    h->sbrMcnt.ensure(z.mcnt);
+    h->sbrProg.ensure(z.prog);
+    h->sbrFlag.ensure(4);
+    sw = SbrWork{h->sbrV.p, h->sbrT.p, h->sbrR.p, h->sbrCnt.p, h->sbrX.p, h->sbrY.p,
+                 h->sbrMp.p, h->sbrM.p, h->sbrMcnt.p, h->sbrAB.p, h->sbrProg.p, h->sbrFlag.p,
+                 h->VW.p, nm, h->dd.p, h->ee.p};
+  }
+  auto reduce = [&]() {
+    if (method == 2)
+      sbr_tridiagonalize(s, (int)nm, h->C.p, nm, sw);
+    else
+      tridiagonalize(s, (int)nm, h->C.p, nm, wk);
+  };
   const char *g = getenv("MBD_TRD_GRAPH");
   if (h->prof_on || (g && g[0] == '0')) {
-    tridiagonalize(s, (int)nm, h->C.p, nm, wk);
+    reduce();
   } else {
-    const std::vector<const void *> key = {(const void *)(intptr_t)nm, h->C.p, h->VW.p,
-                                           h->tau.p, h->dd.p, h->ee.p, h->alph.p, h->pa.p,
-                                           h->pb.p, h->py.p, h->qred.p, h->ps.p};
+    const std::vector<const void *> key = {(const void *)(intptr_t)nm, (const void *)(intptr_t)method,
+                                           h->C.p, h->VW.p, h->tau.p, h->dd.p, h->ee.p, h->alph.p,
+                                           h->pa.p, h->pb.p, h->py.p, h->qred.p, h->ps.p,
+                                           sw.Vn, sw.Tn, sw.Rn, sw.cnt, sw.X, sw.Yp, sw.Mp, sw.M,
+                                           sw.mcnt, sw.AB, sw.prog, sw.flag};
     if (!h->trd_exec || h->trd_key != key) {
       if (h->trd_exec) (void)hipGraphExecDestroy(h->trd_exec);
       h->trd_exec = nullptr;
@@ -488,7 +524,7 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
       hipGraph_t gr = nullptr;
       MBD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
       try {
-        tridiagonalize(s, (int)nm, h->C.p, nm, wk);
+        reduce();
       } catch (const HipError &) {
         (void)hipStreamEndCapture(s, &gr);
         if (gr) (void)hipGraphDestroy(gr);
@@ -502,6 +538,8 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
     }
     MBD_HIP(hipGraphLaunch(h->trd_exec, s));
   }
+  if (method == 2)
+    MBD_HIP(hipMemcpyAsync(&h->eig_flag, sw.flag, sizeof(int), hipMemcpyDeviceToHost, s));
   h->lam.ensure(nm);
   h->e2.ensure(nm);
   h->bounds.ensure(4);
@@ -799,6 +837,10 @@ int mbd_energy(mbd_handle *h, const double *alpha_eff, const double *c6_eff, dou
     fetch(h, reff, h->reff.p, N);
     MBD_HIP(hipStreamSynchronize(h->stream));
     *e_mbd = E[0];
+    if (h->eig_flag) {
+      if (d) d->info_eig = 1;
+      return MBD_ERR_EIG;
+    }
     if (d) {
       d->n_negative = nn;
       d->t_mbd_ms = ms;
@@ -833,6 +875,10 @@ int mbd_run(mbd_handle *h, int mode, double *e_mbd, double *alpha_eff, double *c
     MBD_HIP(hipMemcpyAsync(&nn, h->nneg.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     fetch_scs_outputs(h, alpha_eff, coupled_pol, molpol_eig, c6_eff, c6_mol);
     *e_mbd = E[0];
+    if (h->eig_flag) {
+      if (d) d->info_eig = 1;
+      return MBD_ERR_EIG;
+    }
     if (d) {
       d->n_negative = nn;
       d->t_total_ms = ms_all;
@@ -1214,11 +1260,29 @@ int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w) {
     wk.py = alloc(nbt * nbt * kTrdTile);
     wk.qred = alloc(kTrdQred);
     wk.ps = alloc(nbt + 1);
-    tridiagonalize(nullptr, n, dA, n, wk);
+    int abandoned = 0;
+    if (eig_method(n) == 2) {
+      const SbrSizes z = sbr_sizes(n);
+      auto ialloc = [&](size_t cnt) {
+        int *p = reinterpret_cast<int *>(alloc((cnt + 1) / 2));
+        return p;
+      };
+      SbrWork sw{alloc(z.V), alloc(z.TR), alloc(z.TR), ialloc(z.cnt), alloc(z.X), alloc(z.Yp),
+                 alloc(z.Mp), alloc((size_t)kSbrB * kSbrB), ialloc(z.mcnt), alloc(z.AB),
+                 ialloc(z.prog), ialloc(4), wk.VW, wk.ldv, wk.d, wk.e};
+      sbr_tridiagonalize(nullptr, n, dA, n, sw);
+      MBD_HIP(hipMemcpy(&abandoned, sw.flag, sizeof(int), hipMemcpyDeviceToHost));
+    } else {
+      tridiagonalize(nullptr, n, dA, n, wk);
+    }
     double *e2 = alloc(n), *bounds = alloc(4), *lam = alloc(n);
     tridiag_eigvals(nullptr, n, wk.d, wk.e, e2, bounds, lam);
     MBD_HIP(hipDeviceSynchronize());
     MBD_HIP(hipMemcpy(w, lam, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (abandoned) {
+      for (auto *p : bufs) (void)hipFree(p);
+      return MBD_ERR_EIG;
+    }
   } catch (const HipError &e) {
     for (auto *p : bufs) (void)hipFree(p);
     return fail(nullptr, e);

@@ -62,6 +62,39 @@ struct TrdWork {
   double *ps;     // squared-norm partials of the column, ceil(n/kTrdTile) + 1
 };
 void tridiagonalize(hipStream_t s, int n, double *A, long lda, TrdWork &wk);
+
+// sbr.hip: two-stage reduction (dense -> band of half-width kSbrB -> tridiagonal)
+constexpr int kSbrB = 32;          // band half-width = panel width
+constexpr int kSbrLeaf = 256;      // rows of a TSQR tree node = threads of its workgroup
+constexpr int kSbrFan = kSbrLeaf / kSbrB;  // children per TSQR tree node
+constexpr int kSbrLdb = 2 * kSbrB;  // band storage: diagonals 0 .. 2B-1 of each column
+constexpr int kSbrMaxLev = 8;
+constexpr int kSbrMaxSplit = 8;    // column splits of the symmetric product
+constexpr int kSbrChaseWG = 256;   // workgroups of the bulge-chasing kernel
+struct SbrWork {
+  double *Vn, *Tn, *Rn;  // TSQR nodes: reflectors (kSbrLeaf x B, row-major), T, R
+  int *cnt;              // per-node arrival counters
+  double *X;             // V T, n x B (ld n)
+  double *Yp;            // split partials of A22 X, kSbrMaxSplit x B x n
+  double *Mp;            // partials of X^T A22 X
+  double *M;             // X^T A22 X, B x B
+  int *mcnt;             // arrival counters of the M reduction
+  double *AB;            // band, kSbrLdb x n (column-major)
+  int *prog;             // tasks completed per sweep, n
+  int *flag;             // [0]: bulge chase abandoned (spin limit)
+  double *VW;            // [V | Z | V], n x 3B (ld ldv)
+  long ldv;
+  double *d, *e;         // tridiagonal output
+};
+// element counts of the work buffers for order n
+struct SbrSizes {
+  size_t nodes, V, TR, cnt, X, Yp, Mp, mcnt, AB, prog;
+};
+SbrSizes sbr_sizes(int n);
+void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w);
+// eigen-reduction method for order n: 1 = one-stage (tridiagonalize), 2 = two-stage;
+// MBD_EIG_METHOD=1|2 forces one
+int eig_method(int n);
 // one-time kernel attributes of tridiagonalize (call before capturing it)
 void tridiag_prepare();
 void tridiag_eigvals(hipStream_t s, int n, const double *d, const double *e,

@@ -28,6 +28,9 @@ enum KernelClass {
   K_PCG_SYMV,
   K_PCG_SMALL,
   K_NET,
+  K_SBR_QR,
+  K_SBR_SYMM,
+  K_SBR_CHASE,
   K_NUM_CLASSES
 };
 

@@ -1,0 +1,852 @@
+// Two-stage reduction of the coupled-QHO matrix to tridiagonal form, FP64
+// (the reduction inside PDSYEV('N','L') at UTILS.f90:835-858, eigenvalues only).
+//
+// Stage 1, dense -> band of half-width B = 32, one panel of B columns at a time:
+//   k_sbr_tsqr   QR of the m x B panel below the diagonal block as a tree
+//                (TSQR): each 256-row leaf is factored by one workgroup with
+//                its rows in registers, the R factors of 8 siblings are
+//                stacked and factored by the last sibling to finish;
+//   k_sbr_qform  explicit Q from the tree, then Householder reconstruction
+//                (LU without pivoting of Q S - [I; 0], S = -sign diag(Q))
+//                gives compact-WY reflectors V, T with H = I - V T V^T whose
+//                first B columns are Q S; writes V, X = V T and the band
+//                block S R;
+//   k_sbr_symm   Y = A22 X from the lower triangle on MFMA (FP64 16x16x4),
+//                the columns split over workgroups into partials;
+//   k_sbr_zred   sums the partials, M = X^T Y by a two-level arrival tree;
+//   k_sbr_z      Z = Y - V M / 2;
+//   gemm_nt      A22 -= V Z^T + Z V^T on the lower tiles (MFMA SYR2K).
+// Stage 2, band -> tridiagonal by bulge chasing (k_sbr_chase): sweep s
+// annihilates column s below the subdiagonal with a length-B reflector and
+// chases the bulge down in tasks of B rows; one wavefront per sweep, the
+// sweeps pipelined across a persistent grid: task k of sweep s starts once
+// sweep s - 1 has completed task k + 2, which leaves the two tasks' blocks
+// disjoint.  The band (2B diagonals per column) stays in HBM/L2; a task holds
+// its three 32 x 32 blocks in registers.
+#include <algorithm>
+#include <cstdlib>
+
+#include "common.h"
+#include "kernels.h"
+#include "profile.h"
+
+namespace mbd {
+
+namespace {
+constexpr int B = kSbrB;
+constexpr int NT = kSbrLeaf;
+constexpr int FAN = kSbrFan;
+constexpr int LDB = kSbrLdb;
+constexpr int NW = NT / 64;
+constexpr int kChaseSpin = 1 << 22;  // polls (about 1 us each) before a sweep gives up
+
+struct Tree {
+  int nlev;
+  int L[kSbrMaxLev];    // nodes per level, leaves first
+  int off[kSbrMaxLev];  // index of the first node of each level
+};
+
+Tree make_tree(int m) {
+  Tree t{};
+  int L = ceil_div(m, NT), off = 0, lev = 0;
+  for (;;) {
+    t.L[lev] = L;
+    t.off[lev] = off;
+    off += L;
+    ++lev;
+    if (L == 1) break;
+    L = ceil_div(L, FAN);
+  }
+  t.nlev = lev;
+  return t;
+}
+
+// Reflector scalars from |x|^2 below the head and the head element (LAPACK
+// dlarfg without its rescaling loop): H = I - tau v v^T, v = e_0 + sc x.
+__device__ inline void reflector(double xn2, double alp, double &tau, double &beta, double &sc) {
+  if (xn2 == 0.0) {
+    tau = 0.0;
+    beta = alp;
+    sc = 0.0;
+  } else {
+    beta = -copysign(sqrt(alp * alp + xn2), alp);
+    tau = (beta - alp) / beta;
+    sc = 1.0 / (alp - beta);
+  }
+}
+
+// Sums q[0..31] over the workgroup into S[0..31] (LDS).  Within a wave by
+// recursive halving (after five exchanges lane l holds value l >> 1), then
+// across the waves through LDS.  Two barriers.
+__device__ inline void block_sum32(double (&q)[32], double (*part)[32], double *S) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+#pragma unroll
+  for (int st = 0; st < 5; ++st) {
+    const int h = 16 >> st, mask = 32 >> st;
+    const bool up = (lane & mask) != 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k < h) {
+        double lo = q[k], hi = q[k + h];
+        // opaque copies: keeps the lane-dependent choice a select of register
+        // values (folded into a select of addresses it sends q[] to scratch)
+        asm volatile("" : "+v"(lo), "+v"(hi));
+        const double send = up ? lo : hi;
+        const double keep = up ? hi : lo;
+        q[k] = keep + __shfl_xor(send, mask, 64);
+      }
+    }
+  }
+  const double v = q[0] + __shfl_xor(q[0], 1, 64);
+  if ((lane & 1) == 0) part[wv][lane >> 1] = v;
+  __syncthreads();
+  if (t < 32) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += part[w][t];
+    S[t] = s;
+  }
+  __syncthreads();
+}
+
+// Householder QR of a node of at most NT rows, thread t holding row t in a[]
+// (rows past the node hold zeros).  On return a[] holds R on and above the
+// diagonal and the reflectors below it (unit diagonal implied); Ts holds the
+// upper-triangular T of H_0 ... H_31 = I - V T V^T.  One fused reduction per
+// column: with x the column below the head, sum_t x_t a_t[k] gives |x|^2
+// (k = j), the dots of the reflector with the trailing columns (k > j) and
+// with the earlier reflectors (k < j, for T).
+// Step J of node_qr, J a template parameter so that the register row a[] is
+// only ever indexed by constants.
+template <int J>
+__device__ inline void qr_step(double (&a)[32], double (*part)[32], double *S, double (*rowj)[32],
+                               double (*Ts)[B + 1]) {
+  const int t = threadIdx.x;
+  double *rj = rowj[J & 1];
+  const double x = a[J];
+  const bool below = t > J;
+  double q[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) q[k] = below ? x * a[k] : 0.0;
+  if (t == J) {
+#pragma unroll
+    for (int k = 0; k < 32; ++k) rj[k] = a[k];
+  }
+  block_sum32(q, part, S);
+  double tau, beta, sc;
+  reflector(S[J], rj[J], tau, beta, sc);
+  if (t >= J) {
+    const double tv = tau * ((t == J) ? 1.0 : sc * x);
+#pragma unroll
+    for (int k = J + 1; k < 32; ++k) a[k] -= tv * (rj[k] + sc * S[k]);
+    a[J] = (t == J) ? beta : sc * x;
+  }
+  if (t < J) {
+    double acc = 0.0;
+    for (int l = t; l < J; ++l) acc += Ts[t][l] * (rj[l] + sc * S[l]);
+    Ts[t][J] = -tau * acc;
+  } else if (t == J) {
+    Ts[J][J] = tau;
+  }
+}
+
+template <int J>
+__device__ inline void qr_steps(double (&a)[32], double (*part)[32], double *S, double (*rowj)[32],
+                                double (*Ts)[B + 1]) {
+  qr_step<J>(a, part, S, rowj, Ts);
+  if constexpr (J + 1 < B) qr_steps<J + 1>(a, part, S, rowj, Ts);
+}
+
+__device__ inline void node_qr(double (&a)[32], double (*part)[32], double *S, double (*rowj)[32],
+                               double (*Ts)[B + 1]) {
+  qr_steps<0>(a, part, S, rowj, Ts);
+}
+
+// Reflector element (r, i) of a node stored as in node_qr (unit diagonal,
+// zero above it).
+__device__ inline double vget(const double *V, int r, int i) {
+  return r > i ? V[r * B + i] : (r == i ? 1.0 : 0.0);
+}
+
+}  // namespace
+
+// ---- stage 1 --------------------------------------------------------------
+
+// Tree QR of the panel P (m x w, w <= B; columns w.. are zero).  Grid: one
+// workgroup per leaf; the last of each FAN siblings to finish factors their
+// stacked R factors as the parent node.
+__global__ __launch_bounds__(NT) void k_sbr_tsqr(int m, int w, const double *__restrict__ P,
+                                                 long lda, double *__restrict__ Vn,
+                                                 double *__restrict__ Tn, double *__restrict__ Rn,
+                                                 int *cnt, Tree tr) {
+  __shared__ double part[NW][32];
+  __shared__ double S[32];
+  __shared__ double rowj[2][32];
+  __shared__ double Ts[B][B + 1];
+  __shared__ int last;
+  const int t = threadIdx.x;
+  int lev = 0, idx = blockIdx.x;
+  int nrows = min(NT, m - NT * idx);
+  double a[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k)
+    a[k] = (t < nrows && k < w) ? P[(long)NT * idx + t + (long)k * lda] : 0.0;
+  for (;;) {
+    for (int e = t; e < B * (B + 1); e += NT) (&Ts[0][0])[e] = 0.0;
+    __syncthreads();
+    node_qr(a, part, S, rowj, Ts);
+    __syncthreads();
+    const long node = tr.off[lev] + idx;
+    double *V = Vn + node * NT * B;
+    if (t < nrows) {
+#pragma unroll
+      for (int k = 0; k < 32; ++k) V[t * B + k] = a[k];
+    }
+    for (int e = t; e < B * B; e += NT) Tn[node * B * B + e] = Ts[e / B][e % B];
+    if (t < B) {
+#pragma unroll
+      for (int k = 0; k < 32; ++k) Rn[node * B * B + t * B + k] = (k >= t) ? a[k] : 0.0;
+    }
+    if (lev == tr.nlev - 1) return;
+    __threadfence();
+    __syncthreads();
+    const int par = idx / FAN;
+    const int nch = min(FAN, tr.L[lev] - FAN * par);
+    if (t == 0) last = (atomicAdd(&cnt[tr.off[lev + 1] + par], 1) == nch - 1);
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (t == 0) cnt[tr.off[lev + 1] + par] = 0;  // every sibling has arrived
+    const long c0 = tr.off[lev] + (long)FAN * par;
+    ++lev;
+    idx = par;
+    nrows = B * nch;
+#pragma unroll
+    for (int k = 0; k < 32; ++k)
+      a[k] = (t < nrows) ? Rn[(c0 + t / B) * B * B + (t % B) * B + k] : 0.0;
+  }
+}
+
+// Explicit Q of the panel (rows of this workgroup's leaf), Householder
+// reconstruction and the outputs V (into VW columns 0..B-1 and 2B..3B-1),
+// X = V T and, from workgroup 0, the band block S R (w columns) at Ab.
+__global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__restrict__ Vn,
+                                                  const double *__restrict__ Tn,
+                                                  const double *__restrict__ Rn, Tree tr,
+                                                  double *__restrict__ VW, long ldv,
+                                                  double *__restrict__ X, long ldx,
+                                                  double *__restrict__ Ab, long lda) {
+  __shared__ double Qs[B][B + 1];  // Q block entering the current node
+  __shared__ double M1[B][B + 1];
+  __shared__ double M2[B][B + 1];
+  __shared__ double Tl[B][B + 1];
+  __shared__ double W[B][B + 1];   // Q1 S - I, then its LU factors
+  __shared__ double Tf[B][B + 1];  // T of the reconstructed reflectors
+  __shared__ double sgn[B];
+  __shared__ double rowbuf[NT][B + 1];  // this thread's row of Q, then of V
+  const int t = threadIdx.x;
+  const int leaf = blockIdx.x;
+
+  // M2 = T_node (V_node[0:B]^T Qs) for node `node`
+  auto node_m2 = [&](long node) {
+    const double *V = Vn + node * NT * B;
+    for (int e = t; e < B * B; e += NT) Tl[e / B][e % B] = Tn[node * B * B + e];
+    for (int e = t; e < B * B; e += NT) {
+      const int i = e / B, k = e % B;
+      double s = 0.0;
+      for (int r = i; r < B; ++r) s += vget(V, r, i) * Qs[r][k];
+      M1[i][k] = s;
+    }
+    __syncthreads();
+    for (int e = t; e < B * B; e += NT) {
+      const int i = e / B, k = e % B;
+      double s = 0.0;
+      for (int l = i; l < B; ++l) s += Tl[i][l] * M1[l][k];
+      M2[i][k] = s;
+    }
+    __syncthreads();
+  };
+  // Qs <- rows [B c, B c + B) of H_node [Qs; 0]
+  auto node_down = [&](long node, int c) {
+    node_m2(node);
+    const double *V = Vn + node * NT * B;
+    double nv[B * B / NT];
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q, r = e / B, k = e % B;
+      double s = (c == 0) ? Qs[r][k] : 0.0;
+      for (int i = 0; i < B; ++i) s -= vget(V, B * c + r, i) * M2[i][k];
+      nv[q] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q;
+      Qs[e / B][e % B] = nv[q];
+    }
+    __syncthreads();
+  };
+  // Qs <- the B x B block of the tree's Q that enters leaf `target`
+  auto walk = [&](int target) {
+    for (int e = t; e < B * B; e += NT) Qs[e / B][e % B] = (e / B == e % B) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int lev = tr.nlev - 1; lev >= 1; --lev) {
+      int span = 1;
+      for (int l = 0; l < lev - 1; ++l) span *= FAN;  // leaves under a child
+      const int pidx = target / (span * FAN), c = (target / span) % FAN;
+      node_down(tr.off[lev] + pidx, c);
+    }
+  };
+
+  // Q1 = top B rows of Q (leaf 0), every workgroup
+  walk(0);
+  node_m2(tr.off[0]);
+  {
+    const double *V = Vn + (long)tr.off[0] * NT * B;
+    for (int e = t; e < B * B; e += NT) {
+      const int r = e / B, k = e % B;
+      double s = Qs[r][k];
+      for (int i = 0; i <= r; ++i) s -= vget(V, r, i) * M2[i][k];
+      W[r][k] = s;
+    }
+  }
+  __syncthreads();
+  if (t < B) sgn[t] = (W[t][t] >= 0.0) ? -1.0 : 1.0;
+  __syncthreads();
+  for (int e = t; e < B * B; e += NT) {
+    const int r = e / B, k = e % B;
+    W[r][k] = W[r][k] * sgn[k] - (r == k ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  // LU without pivoting (|pivot| >= 1 by the choice of signs)
+  for (int kk = 0; kk < B - 1; ++kk) {
+    if (t > kk && t < B) W[t][kk] /= W[kk][kk];
+    __syncthreads();
+    for (int e = t; e < B * B; e += NT) {
+      const int r = e / B, c = e % B;
+      if (r > kk && c > kk) W[r][c] -= W[r][kk] * W[kk][c];
+    }
+    __syncthreads();
+  }
+  // Tf = -U Y1^{-T}: columns of Z = Y1^{-T} (unit upper) into M1
+  if (t < B) {
+    const int k = t;
+    for (int i = B - 1; i > k; --i) M1[i][k] = 0.0;
+    M1[k][k] = 1.0;
+    for (int i = k - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int l = i + 1; l <= k; ++l) s += W[l][i] * M1[l][k];
+      M1[i][k] = -s;
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < B * B; e += NT) {
+    const int i = e / B, k = e % B;
+    double s = 0.0;
+    for (int l = i; l <= k; ++l) s += W[i][l] * M1[l][k];
+    Tf[i][k] = (i <= k) ? -s : 0.0;
+  }
+  __syncthreads();
+
+  // rows of this leaf
+  walk(leaf);
+  const long lnode = tr.off[0] + leaf;
+  node_m2(lnode);
+  const int nrows = min(NT, m - NT * leaf);
+  if (t < nrows) {
+    const double *V = Vn + lnode * NT * B;
+    const int g = NT * leaf + t;
+    double *y = rowbuf[t];
+    if (g < B) {
+      for (int k = 0; k < B; ++k) y[k] = (k < g) ? W[g][k] : (k == g ? 1.0 : 0.0);
+    } else {
+      double vrow[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) vrow[i] = vget(V, t, i);
+      for (int k = 0; k < B; ++k) {
+        double s = (t < B) ? Qs[t][k] : 0.0;
+#pragma unroll
+        for (int i = 0; i < B; ++i) s -= vrow[i] * M2[i][k];
+        y[k] = s * sgn[k];
+      }
+      // y U = q S in place (U upper: W on and above the diagonal)
+      for (int k = 0; k < B; ++k) {
+        double s = y[k];
+        for (int i = 0; i < k; ++i) s -= y[i] * W[i][k];
+        y[k] = s / W[k][k];
+      }
+    }
+    for (int k = 0; k < B; ++k) {
+      double s = 0.0;
+      for (int i = 0; i <= k; ++i) s += y[i] * Tf[i][k];
+      VW[g + (long)k * ldv] = y[k];
+      VW[g + (long)(2 * B + k) * ldv] = y[k];
+      X[g + (long)k * ldx] = s;
+    }
+  }
+  if (leaf == 0 && t < B) {
+    const double *R = Rn + (long)tr.off[tr.nlev - 1] * B * B;
+    for (int k = 0; k < w; ++k) Ab[t + (long)k * lda] = (k >= t) ? sgn[t] * R[t * B + k] : 0.0;
+  }
+}
+
+namespace {
+constexpr int SY_R = 64, SY_K = 16;
+}
+
+// Partial Y = A22[rows, cols] X[cols] for row block blockIdx.x (64 rows) and
+// column range blockIdx.y (kchunk columns), A22 read from its lower triangle.
+// Wave v owns rows 16 v .. 16 v + 15 and both 16-column halves of the output.
+__global__ __launch_bounds__(256) void k_sbr_symm(int m, const double *__restrict__ A, long lda,
+                                                  const double *__restrict__ X, long ldx,
+                                                  double *__restrict__ Yp, long ldy, int kchunk) {
+  __shared__ double As[2][SY_K][SY_R + 1];
+  __shared__ double Xs[2][SY_K][B + 1];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 15, lk = lane >> 4;
+  const int r0 = blockIdx.x * SY_R;
+  const int k0 = blockIdx.y * kchunk, k1 = min(m, k0 + kchunk);
+  v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  double ra[4], rx[2];
+  bool up_cur = false;
+  auto gload = [&](int kc) {
+    const bool upper = kc >= r0 + SY_R;  // chunk right of the row block: A22[r][c] = A[c][r]
+    up_cur = upper;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = t + 256 * i;
+      const int rr = upper ? (e >> 4) : (e & 63), kk = upper ? (e & 15) : (e >> 6);
+      const int gr = r0 + rr, gc = kc + kk;
+      double v = 0.0;
+      if (gr < m && gc < k1) v = (gr >= gc) ? A[gr + (long)gc * lda] : A[gc + (long)gr * lda];
+      ra[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int e = t + 256 * i, kk = e & 15, cc = e >> 4;
+      const int gc = kc + kk;
+      rx[i] = (gc < k1) ? X[gc + (long)cc * ldx] : 0.0;
+    }
+  };
+  auto sstore = [&](int buf) {
+    const bool upper = up_cur;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = t + 256 * i;
+      const int rr = upper ? (e >> 4) : (e & 63), kk = upper ? (e & 15) : (e >> 6);
+      As[buf][kk][rr] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int e = t + 256 * i, kk = e & 15, cc = e >> 4;
+      Xs[buf][kk][cc] = rx[i];
+    }
+  };
+  const int nk = (k1 > k0) ? (k1 - k0 + SY_K - 1) / SY_K : 0;
+  if (nk > 0) {
+    gload(k0);
+    sstore(0);
+  }
+  __syncthreads();
+  for (int it = 0; it < nk; ++it) {
+    const int cur = it & 1;
+    if (it + 1 < nk) gload(k0 + (it + 1) * SY_K);
+#pragma unroll
+    for (int kk = 0; kk < SY_K; kk += 4) {
+      const double a = As[cur][kk + lk][16 * wv + lr];
+      const double b0 = Xs[cur][kk + lk][lr], b1 = Xs[cur][kk + lk][16 + lr];
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+    }
+    if (it + 1 < nk) sstore(cur ^ 1);
+    __syncthreads();
+  }
+  double *Y = Yp + (long)blockIdx.y * B * ldy;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int row = r0 + 16 * wv + lk + 4 * v;
+    if (row < m) {
+      Y[row + (long)lr * ldy] = acc0[v];
+      Y[row + (long)(16 + lr) * ldy] = acc1[v];
+    }
+  }
+}
+
+// Y = sum of the split partials (written over split 0), and M = X^T Y: each
+// 64-row block's product goes to Mp, groups of 16 blocks are summed by their
+// last arrival, the group sums by the last group.
+__global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__restrict__ Yp,
+                                                  long ldy, const double *__restrict__ X,
+                                                  long ldx, double *__restrict__ Mp, int *mcnt,
+                                                  double *__restrict__ M) {
+  constexpr int G = 16;
+  __shared__ double Ys[64][B + 1], Xs[64][B + 1];
+  __shared__ int last;
+  const int t = threadIdx.x, I = blockIdx.x, nb = gridDim.x;
+  const int r = t & 63, cg = t >> 6;
+  const int row = 64 * I + r;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = 8 * cg + j;
+    double s = 0.0, xv = 0.0;
+    if (row < m) {
+      for (int q = 0; q < nsplit; ++q) s += Yp[(long)q * B * ldy + row + (long)c * ldy];
+      Yp[row + (long)c * ldy] = s;
+      xv = X[row + (long)c * ldx];
+    }
+    Ys[r][c] = s;
+    Xs[r][c] = xv;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = t + 256 * i, a = e >> 5, b = e & 31;
+    double s = 0.0;
+    for (int rr = 0; rr < 64; ++rr) s += Xs[rr][a] * Ys[rr][b];
+    Mp[(long)I * B * B + e] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  const int grp = I / G, ng = (nb + G - 1) / G, gsz = min(G, nb - grp * G);
+  if (t == 0) last = (atomicAdd(&mcnt[1 + grp], 1) == gsz - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (t == 0) mcnt[1 + grp] = 0;
+  double *Mg = Mp + (long)nb * B * B;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = t + 256 * i;
+    double s = 0.0;
+    for (int q = 0; q < gsz; ++q) s += Mp[(long)(grp * G + q) * B * B + e];
+    Mg[(long)grp * B * B + e] = s;
+  }
+  __threadfence();
+  __syncthreads();
+  if (t == 0) last = (atomicAdd(&mcnt[0], 1) == ng - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (t == 0) mcnt[0] = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = t + 256 * i;
+    double s = 0.0;
+    for (int g = 0; g < ng; ++g) s += Mg[(long)g * B * B + e];
+    M[e] = s;
+  }
+}
+
+// Z = Y - V M / 2 into VW columns B..2B-1 (one thread per row).
+__global__ __launch_bounds__(256) void k_sbr_z(int m, const double *__restrict__ Y, long ldy,
+                                               const double *__restrict__ M,
+                                               double *__restrict__ VW, long ldv) {
+  __shared__ double Ms[B][B + 1];
+  const int t = threadIdx.x;
+  for (int e = t; e < B * B; e += 256) Ms[e / B][e % B] = M[e];
+  __syncthreads();
+  const int row = blockIdx.x * 256 + t;
+  if (row >= m) return;
+  double v[B];
+#pragma unroll
+  for (int i = 0; i < B; ++i) v[i] = VW[row + (long)i * ldv];
+#pragma unroll 4
+  for (int k = 0; k < B; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < B; ++i) s += v[i] * Ms[i][k];
+    VW[row + (long)(B + k) * ldv] = Y[row + (long)k * ldy] - 0.5 * s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sbr_zero(int ncnt, int *cnt, int nm, int *mcnt) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e < ncnt) cnt[e] = 0;
+  if (e < nm) mcnt[e] = 0;
+}
+
+// ---- stage 2 --------------------------------------------------------------
+
+// Band storage AB[c * LDB + d] = A[c + d][c] for d <= B (zero for d > B and
+// past the matrix); clears the sweep progress counters and the abort flag.
+__global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restrict__ A, long lda,
+                                                  double *__restrict__ AB, int *prog, int *flag) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e < (long)n * LDB) {
+    const int c = (int)(e / LDB), d = (int)(e % LDB);
+    AB[e] = (d <= B && c + d < n) ? A[(c + d) + (long)c * lda] : 0.0;
+  }
+  if (e < n) prog[e] = 0;
+  if (e < 4) flag[e] = 0;
+}
+
+namespace {
+// Sum over the 32 lanes of a half-wave of q[0..15]; afterwards lane l holds
+// value ((l & 31) >> 1).
+__device__ inline double half_sum16(double (&q)[16], int lane) {
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const int h = 8 >> st, mask = 16 >> st;
+    const bool up = (lane & mask) != 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k < h) {
+        double lo = q[k], hi = q[k + h];
+        asm volatile("" : "+v"(lo), "+v"(hi));
+        const double send = up ? lo : hi;
+        const double keep = up ? hi : lo;
+        q[k] = keep + __shfl_xor(send, mask, 64);
+      }
+    }
+  }
+  return q[0] + __shfl_xor(q[0], 1, 64);
+}
+
+// Waits until *p >= need (acquire).  Gives up (and raises the abort flag for
+// every other sweep) after kChaseSpin polls, or when another sweep gave up.
+__device__ inline bool chase_wait(const int *p, int need, int *flag) {
+  int it = 0;
+  for (;;) {
+    const int v = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (v >= need) break;
+    const int ab = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (ab != 0) return false;
+    if (++it > kChaseSpin) {
+      __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __threadfence();
+  return true;
+}
+}  // namespace
+
+// Bulge chasing, one wavefront per sweep (sweeps s = blockIdx.x + k gridDim.x).
+// Task k of sweep s works on rows I = [p, p + B), p = s + 1 + k B: the
+// reflector from the first column of L = A[I, previous block] (k = 0: column
+// s), applied from the left to L, from both sides to D = A[I, I] and from the
+// right to Bn = A[next B rows, I], which becomes the next task's L.  Lane
+// (h, c) = (lane >> 5, lane & 31) holds column c, rows 16 h .. 16 h + 15 of
+// each block.
+__global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB, int *prog,
+                                                  int *flag) {
+  __shared__ double vs[B], ys[B], us[B];
+  const int lane = threadIdx.x, c = lane & 31, h = lane >> 5;
+  double L[16], D[16], Bn[16];
+  for (int s = blockIdx.x; s <= n - 3; s += gridDim.x) {
+    const int nt = (n - 3 - s) / B + 1;
+    const int ntp = (s > 0) ? (n - 2 - s) / B + 1 : 0;
+    for (int k = 0; k < nt; ++k) {
+      if (s > 0 && !chase_wait(&prog[s - 1], min(k + 3, ntp), flag)) return;
+      const int p = s + 1 + B * k;
+      const int l = min(B, n - p);
+      const int l2 = max(0, min(B, n - p - B));
+      if (k == 0) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int r = 16 * h + i;
+          L[i] = (c == 0 && r < l) ? AB[(long)s * LDB + 1 + r] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int r = 16 * h + i;
+        D[i] = (r < l && c < l)
+                   ? (r >= c ? AB[(long)(p + c) * LDB + (r - c)] : AB[(long)(p + r) * LDB + (c - r)])
+                   : 0.0;
+        Bn[i] = (r < l2 && c < l) ? AB[(long)(p + c) * LDB + B + r - c] : 0.0;
+      }
+      // reflector from column 0 of L (rows 0 .. l-1)
+      double part = 0.0;
+      if (c == 0) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+          if (16 * h + i >= 1) part += L[i] * L[i];
+      }
+      const double xn2 = __shfl(part, 0, 64) + __shfl(part, 32, 64);
+      const double alp = __shfl(L[0], 0, 64);
+      double tau, beta, sc;
+      reflector(xn2, alp, tau, beta, sc);
+      if (c == 0) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int r = 16 * h + i;
+          vs[r] = (r == 0) ? 1.0 : sc * L[i];
+        }
+      }
+      __syncthreads();
+      double vr[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) vr[i] = vs[16 * h + i];
+      const double vc = vs[c];
+      // left: L -= tau v (v^T L)
+      double wp = 0.0;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) wp += vr[i] * L[i];
+      const double wc = wp + __shfl_xor(wp, 32, 64);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) L[i] -= tau * vr[i] * wc;
+      if (c == 0) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) L[i] = (16 * h + i == 0) ? beta : 0.0;
+      }
+      // y = tau D v and u = Bn v (row sums over the half-wave)
+      double q[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) q[i] = D[i] * vc;
+      const double ysum = half_sum16(q, lane);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) q[i] = Bn[i] * vc;
+      const double usum = half_sum16(q, lane);
+      if ((c & 1) == 0) {
+        ys[16 * h + (c >> 1)] = tau * ysum;
+        us[16 * h + (c >> 1)] = usum;
+      }
+      __syncthreads();
+      double yr[16];
+      double kp = 0.0;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        yr[i] = ys[16 * h + i];
+        kp += vr[i] * yr[i];
+      }
+      const double K = 0.5 * tau * (kp + __shfl_xor(kp, 32, 64));
+      const double yc = ys[c] - K * vc;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        D[i] -= vr[i] * yc + (yr[i] - K * vr[i]) * vc;
+        Bn[i] -= tau * us[16 * h + i] * vc;
+      }
+      // stores: L, the lower half of D, and Bn when no later task of this sweep takes it
+      if (k == 0) {
+        if (c == 0) {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const int r = 16 * h + i;
+            if (r < l) AB[(long)s * LDB + 1 + r] = L[i];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int r = 16 * h + i;
+          if (r < l) AB[(long)(p - B + c) * LDB + B + r - c] = L[i];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int r = 16 * h + i;
+        if (r < l && c < l && r >= c) AB[(long)(p + c) * LDB + (r - c)] = D[i];
+      }
+      if (k + 1 == nt && l2 > 0) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int r = 16 * h + i;
+          if (r < l2 && c < l) AB[(long)(p + c) * LDB + B + r - c] = Bn[i];
+        }
+      }
+      __syncthreads();  // every lane's stores issued before the release
+      __threadfence();
+      if (lane == 0) __hip_atomic_store(&prog[s], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) L[i] = Bn[i];
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sbr_de(int n, const double *__restrict__ AB,
+                                                double *__restrict__ d, double *__restrict__ e) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    d[i] = AB[(long)i * LDB];
+    if (i < n - 1) e[i] = AB[(long)i * LDB + 1];
+  }
+}
+
+// ---- host -----------------------------------------------------------------
+
+SbrSizes sbr_sizes(int n) {
+  SbrSizes z{};
+  const int L0 = std::max(1, ceil_div(n, NT));
+  z.nodes = 2 * (size_t)L0 + kSbrMaxLev;
+  z.V = z.nodes * NT * B;
+  z.TR = z.nodes * B * B;
+  z.cnt = z.nodes;
+  z.X = (size_t)n * B;
+  z.Yp = (size_t)kSbrMaxSplit * B * n;
+  const int nb = ceil_div(n, 64);
+  z.Mp = ((size_t)nb + ceil_div(nb, 16) + 1) * B * B;
+  z.mcnt = (size_t)ceil_div(nb, 16) + 2;
+  z.AB = (size_t)n * LDB;
+  z.prog = (size_t)n;
+  return z;
+}
+
+int eig_method(int n) {
+  const char *e = getenv("MBD_EIG_METHOD");
+  if (e && (e[0] == '1' || e[0] == '2')) return e[0] - '0';
+  return n >= 1024 ? 2 : 1;
+}
+
+void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
+  const SbrSizes z = sbr_sizes(n);
+  hipLaunchKernelGGL(k_sbr_zero, dim3(ceil_div((long)std::max(z.cnt, z.mcnt), 256)), dim3(256),
+                     0, s, (int)z.cnt, w.cnt, (int)z.mcnt, w.mcnt);
+  MBD_LAUNCH_CHECK();
+  int c = 0, pw = (n % B) ? n % B : B;
+  while (c + pw < n) {
+    const int r0 = c + pw, m = n - r0;
+    const Tree tr = make_tree(m);
+    {
+      ProfScope ps(K_SBR_QR, s, 8.0 * 2.0 * m * B, 4.0 * (double)m * B * B);
+      hipLaunchKernelGGL(k_sbr_tsqr, dim3(tr.L[0]), dim3(NT), 0, s, m, pw,
+                         A + r0 + (long)c * lda, lda, w.Vn, w.Tn, w.Rn, w.cnt, tr);
+      MBD_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_sbr_qform, dim3(tr.L[0]), dim3(NT), 0, s, m, pw, w.Vn, w.Tn, w.Rn,
+                         tr, w.VW, w.ldv, w.X, (long)n, A + r0 + (long)c * lda, lda);
+      MBD_LAUNCH_CHECK();
+    }
+    const int nb = ceil_div(m, 64);
+    int ks = std::max(1, std::min(kSbrMaxSplit, 512 / nb));
+    const int kchunk = ceil_div(ceil_div(m, ks), SY_K) * SY_K;
+    ks = ceil_div(m, kchunk);
+    {
+      // algorithmic: the lower triangle once; flops of the full product
+      ProfScope ps(K_SBR_SYMM, s, 8.0 * 0.5 * m * (m + 1.0), 2.0 * (double)m * m * B);
+      hipLaunchKernelGGL(k_sbr_symm, dim3(nb, ks), dim3(256), 0, s, m, A + r0 + (long)r0 * lda,
+                         lda, w.X, (long)n, w.Yp, (long)n, kchunk);
+      MBD_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_sbr_zred, dim3(nb), dim3(256), 0, s, m, ks, w.Yp, (long)n, w.X,
+                       (long)n, w.Mp, w.mcnt, w.M);
+    MBD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sbr_z, dim3(ceil_div(m, 256)), dim3(256), 0, s, m, w.Yp, (long)n, w.M,
+                       w.VW, w.ldv);
+    MBD_LAUNCH_CHECK();
+    {
+      ProfScope ps(K_TRD_SYR2K, s, 0.0, 2.0 * B * (double)m * (m + 1.0));
+      gemm_nt(s, true, m, m, 2 * B, -1.0, w.VW, (int)w.ldv, 0, w.VW + (long)B * w.ldv,
+              (int)w.ldv, 0, 1.0, A + r0 + (long)r0 * lda, (int)lda, 0, 1);
+    }
+    c = r0;
+    pw = B;
+  }
+  hipLaunchKernelGGL(k_sbr_band, dim3(ceil_div((long)n * LDB, 256)), dim3(256), 0, s, n, A, lda,
+                     w.AB, w.prog, w.flag);
+  MBD_LAUNCH_CHECK();
+  if (n >= 3) {
+    // about n^2 / (2B) tasks, each loading two B x B blocks and storing one and
+    // a half (3.5 B^2 doubles) and doing about 12 B^2 flops
+    ProfScope ps(K_SBR_CHASE, s, 8.0 * 3.5 * B * 0.5 * (double)n * n, 12.0 * B * 0.5 * (double)n * n);
+    hipLaunchKernelGGL(k_sbr_chase, dim3(std::min(n - 2, kSbrChaseWG)), dim3(64), 0, s, n, w.AB,
+                       w.prog, w.flag);
+    MBD_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_sbr_de, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w.AB, w.d, w.e);
+  MBD_LAUNCH_CHECK();
+}
+
+}  // namespace mbd
