@@ -82,15 +82,17 @@ struct SbrWork {
   double *AB;            // band, kSbrLdb x n (column-major)
   int *prog;             // tasks completed per sweep, n
   int *flag;             // [0]: bulge chase abandoned (spin limit)
+  unsigned long long *mail;  // reflector heads passed between sweeps, n x sbr_mail_tasks(n)
   double *VW;            // [V | Z | V], n x 3B (ld ldv)
   long ldv;
   double *d, *e;         // tridiagonal output
 };
 // element counts of the work buffers for order n
 struct SbrSizes {
-  size_t nodes, V, TR, cnt, X, Yp, Mp, mcnt, AB, prog;
+  size_t nodes, V, TR, cnt, X, Yp, Mp, mcnt, AB, prog, mail;
 };
 SbrSizes sbr_sizes(int n);
+int sbr_mail_tasks(int n);
 void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w);
 // eigen-reduction method for order n: 1 = one-stage (tridiagonalize), 2 = two-stage;
 // MBD_EIG_METHOD=1|2 forces one

@@ -142,8 +142,10 @@ __device__ inline void qr_step(double (&a)[32], double (*part)[32], double *S, d
     a[J] = (t == J) ? beta : sc * x;
   }
   if (t < J) {
+    // T[t][J] = -tau sum_{l=t}^{J-1} T[t][l] (V[:, l]^T v), unrolled over l
     double acc = 0.0;
-    for (int l = t; l < J; ++l) acc += Ts[t][l] * (rj[l] + sc * S[l]);
+#pragma unroll
+    for (int l = 0; l < J; ++l) acc += (l >= t) ? Ts[t][l] * (rj[l] + sc * S[l]) : 0.0;
     Ts[t][J] = -tau * acc;
   } else if (t == J) {
     Ts[J][J] = tau;
@@ -240,6 +242,7 @@ __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__
   __shared__ double M1[B][B + 1];
   __shared__ double M2[B][B + 1];
   __shared__ double Tl[B][B + 1];
+  __shared__ double Vs[B][B + 1];  // 32 reflector rows of the current node (unit-lower form)
   __shared__ double W[B][B + 1];   // Q1 S - I, then its LU factors
   __shared__ double Tf[B][B + 1];  // T of the reconstructed reflectors
   __shared__ double sgn[B];
@@ -250,18 +253,24 @@ __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__
   // M2 = T_node (V_node[0:B]^T Qs) for node `node`
   auto node_m2 = [&](long node) {
     const double *V = Vn + node * NT * B;
-    for (int e = t; e < B * B; e += NT) Tl[e / B][e % B] = Tn[node * B * B + e];
+    for (int e = t; e < B * B; e += NT) {
+      Tl[e / B][e % B] = Tn[node * B * B + e];
+      Vs[e / B][e % B] = vget(V, e / B, e % B);
+    }
+    __syncthreads();
     for (int e = t; e < B * B; e += NT) {
       const int i = e / B, k = e % B;
       double s = 0.0;
-      for (int r = i; r < B; ++r) s += vget(V, r, i) * Qs[r][k];
+#pragma unroll 8
+      for (int r = 0; r < B; ++r) s += Vs[r][i] * Qs[r][k];
       M1[i][k] = s;
     }
     __syncthreads();
     for (int e = t; e < B * B; e += NT) {
       const int i = e / B, k = e % B;
       double s = 0.0;
-      for (int l = i; l < B; ++l) s += Tl[i][l] * M1[l][k];
+#pragma unroll 8
+      for (int l = 0; l < B; ++l) s += Tl[i][l] * M1[l][k];
       M2[i][k] = s;
     }
     __syncthreads();
@@ -270,12 +279,17 @@ __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__
   auto node_down = [&](long node, int c) {
     node_m2(node);
     const double *V = Vn + node * NT * B;
+    if (c > 0) {
+      for (int e = t; e < B * B; e += NT) Vs[e / B][e % B] = V[B * B * c + e];
+      __syncthreads();
+    }
     double nv[B * B / NT];
 #pragma unroll
     for (int q = 0; q < B * B / NT; ++q) {
       const int e = t + NT * q, r = e / B, k = e % B;
       double s = (c == 0) ? Qs[r][k] : 0.0;
-      for (int i = 0; i < B; ++i) s -= vget(V, B * c + r, i) * M2[i][k];
+#pragma unroll 8
+      for (int i = 0; i < B; ++i) s -= Vs[r][i] * M2[i][k];
       nv[q] = s;
     }
     __syncthreads();
@@ -301,14 +315,12 @@ __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__
   // Q1 = top B rows of Q (leaf 0), every workgroup
   walk(0);
   node_m2(tr.off[0]);
-  {
-    const double *V = Vn + (long)tr.off[0] * NT * B;
-    for (int e = t; e < B * B; e += NT) {
-      const int r = e / B, k = e % B;
-      double s = Qs[r][k];
-      for (int i = 0; i <= r; ++i) s -= vget(V, r, i) * M2[i][k];
-      W[r][k] = s;
-    }
+  for (int e = t; e < B * B; e += NT) {
+    const int r = e / B, k = e % B;
+    double s = Qs[r][k];
+#pragma unroll 8
+    for (int i = 0; i < B; ++i) s -= Vs[r][i] * M2[i][k];
+    W[r][k] = s;
   }
   __syncthreads();
   if (t < B) sgn[t] = (W[t][t] >= 0.0) ? -1.0 : 1.0;
@@ -566,19 +578,6 @@ __global__ __launch_bounds__(256) void k_sbr_zero(int ncnt, int *cnt, int nm, in
 
 // ---- stage 2 --------------------------------------------------------------
 
-// Band storage AB[c * LDB + d] = A[c + d][c] for d <= B (zero for d > B and
-// past the matrix); clears the sweep progress counters and the abort flag.
-__global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restrict__ A, long lda,
-                                                  double *__restrict__ AB, int *prog, int *flag) {
-  const long e = (long)blockIdx.x * 256 + threadIdx.x;
-  if (e < (long)n * LDB) {
-    const int c = (int)(e / LDB), d = (int)(e % LDB);
-    AB[e] = (d <= B && c + d < n) ? A[(c + d) + (long)c * lda] : 0.0;
-  }
-  if (e < n) prog[e] = 0;
-  if (e < 4) flag[e] = 0;
-}
-
 namespace {
 // Sum over the 32 lanes of a half-wave of q[0..15]; afterwards lane l holds
 // value ((l & 31) >> 1).
@@ -601,14 +600,28 @@ __device__ inline double half_sum16(double (&q)[16], int lane) {
   return q[0] + __shfl_xor(q[0], 1, 64);
 }
 
-// Waits until *p >= need (acquire).  Gives up (and raises the abort flag for
-// every other sweep) after kChaseSpin polls, or when another sweep gave up.
+// Band elements move between workgroups on different XCDs, whose L2 caches are
+// not coherent with each other: every band access is an agent-scope atomic
+// (coherent at the device level, no L2 write-back or invalidate needed), and a
+// task's completion is published only after all its stores have completed.
+__device__ inline double ld_band(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_band(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Empty mailbox slot: a signalling NaN, which no arithmetic result can be.
+constexpr unsigned long long kMailEmpty = 0x7FF4000000000BADull;
+
+// Polls until *p >= need.  Gives up (and raises the abort flag for every other
+// sweep) after kChaseSpin polls, or when another sweep gave up.
 __device__ inline bool chase_wait(const int *p, int need, int *flag) {
   int it = 0;
   for (;;) {
     const int v = __builtin_amdgcn_readfirstlane(
         __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    if (v >= need) break;
+    if (v >= need) return true;
     const int ab = __builtin_amdgcn_readfirstlane(
         __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     if (ab != 0) return false;
@@ -618,20 +631,60 @@ __device__ inline bool chase_wait(const int *p, int need, int *flag) {
     }
     __builtin_amdgcn_s_sleep(1);
   }
-  __threadfence();
-  return true;
+}
+
+// Polls a mailbox slot until it holds a value (same limits as chase_wait).
+__device__ inline bool mail_wait(const unsigned long long *p, double &v, int *flag) {
+  int it = 0;
+  for (;;) {
+    const unsigned long long u = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (u != kMailEmpty) {
+      v = __longlong_as_double((long long)u);
+      return true;
+    }
+    const int ab = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (ab != 0) return false;
+    if (++it > kChaseSpin) {
+      __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
 }
 }  // namespace
+
+// Band storage AB[c * LDB + d] = A[c + d][c] for d <= B (zero for d > B and
+// past the matrix); clears the sweep progress counters, the abort flag and
+// the mailboxes.
+__global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restrict__ A, long lda,
+                                                  double *__restrict__ AB, int *prog, int *flag,
+                                                  unsigned long long *mail, long nmail) {
+  const long stride = (long)gridDim.x * 256;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < (long)n * LDB || e < nmail; e += stride) {
+    if (e < (long)n * LDB) {
+      const int c = (int)(e / LDB), d = (int)(e % LDB);
+      AB[e] = (d <= B && c + d < n) ? A[(c + d) + (long)c * lda] : 0.0;
+    }
+    if (e < nmail) mail[e] = kMailEmpty;
+    if (e < n) prog[e] = 0;
+    if (e < 4) flag[e] = 0;
+  }
+}
 
 // Bulge chasing, one wavefront per sweep (sweeps s = blockIdx.x + k gridDim.x).
 // Task k of sweep s works on rows I = [p, p + B), p = s + 1 + k B: the
 // reflector from the first column of L = A[I, previous block] (k = 0: column
 // s), applied from the left to L, from both sides to D = A[I, I] and from the
-// right to Bn = A[next B rows, I], which becomes the next task's L.  Lane
-// (h, c) = (lane >> 5, lane & 31) holds column c, rows 16 h .. 16 h + 15 of
-// each block.
+// right to Bn = A[next B rows, I], which stays in registers as the next
+// task's L.  Lane (h, c) = (lane >> 5, lane & 31) holds column c, rows
+// 16 h .. 16 h + 15 of each block.
+// Ordering with sweep s - 1: task k starts once s - 1 has completed task
+// k + 1; the one element it shares with task k + 2 of s - 1 (that task's
+// beta, the head of its L) arrives through a mailbox as soon as s - 1 has
+// formed the reflector, and task k + 2 leaves that element to sweep s.
 __global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB, int *prog,
-                                                  int *flag) {
+                                                  unsigned long long *mail, int mt, int *flag) {
   __shared__ double vs[B], ys[B], us[B];
   const int lane = threadIdx.x, c = lane & 31, h = lane >> 5;
   double L[16], D[16], Bn[16];
@@ -639,7 +692,7 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB
     const int nt = (n - 3 - s) / B + 1;
     const int ntp = (s > 0) ? (n - 2 - s) / B + 1 : 0;
     for (int k = 0; k < nt; ++k) {
-      if (s > 0 && !chase_wait(&prog[s - 1], min(k + 3, ntp), flag)) return;
+      if (s > 0 && !chase_wait(&prog[s - 1], min(k + 2, ntp), flag)) return;
       const int p = s + 1 + B * k;
       const int l = min(B, n - p);
       const int l2 = max(0, min(B, n - p - B));
@@ -647,28 +700,33 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           const int r = 16 * h + i;
-          L[i] = (c == 0 && r < l) ? AB[(long)s * LDB + 1 + r] : 0.0;
+          L[i] = (c == 0 && r < l) ? ld_band(&AB[(long)s * LDB + 1 + r]) : 0.0;
         }
       }
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int r = 16 * h + i;
-        D[i] = (r < l && c < l)
-                   ? (r >= c ? AB[(long)(p + c) * LDB + (r - c)] : AB[(long)(p + r) * LDB + (c - r)])
-                   : 0.0;
-        Bn[i] = (r < l2 && c < l) ? AB[(long)(p + c) * LDB + B + r - c] : 0.0;
-      }
-      // reflector from column 0 of L (rows 0 .. l-1)
+      // |x|^2 below the head of column 0 of L; consumed before the D / Bn
+      // loads are issued, so the reflector never waits for them
       double part = 0.0;
       if (c == 0) {
 #pragma unroll
         for (int i = 0; i < 16; ++i)
           if (16 * h + i >= 1) part += L[i] * L[i];
       }
+      asm volatile("" : "+v"(part)::"memory");
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int r = 16 * h + i;
+        D[i] = (r < l && c < l) ? (r >= c ? ld_band(&AB[(long)(p + c) * LDB + (r - c)])
+                                          : ld_band(&AB[(long)(p + r) * LDB + (c - r)]))
+                                : 0.0;
+        Bn[i] = (r < l2 && c < l) ? ld_band(&AB[(long)(p + c) * LDB + B + r - c]) : 0.0;
+      }
       const double xn2 = __shfl(part, 0, 64) + __shfl(part, 32, 64);
       const double alp = __shfl(L[0], 0, 64);
       double tau, beta, sc;
       reflector(xn2, alp, tau, beta, sc);
+      if (k >= 2 && lane == 0)
+        __hip_atomic_store(&mail[(long)s * mt + k], (unsigned long long)__double_as_longlong(beta),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (c == 0) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -691,6 +749,12 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB
       if (c == 0) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) L[i] = (16 * h + i == 0) ? beta : 0.0;
+      }
+      // the head of task k + 2's L of sweep s - 1 (last row, last column of Bn)
+      if (s > 0 && k + 2 < ntp) {
+        double bm = 0.0;
+        if (!mail_wait(&mail[(long)(s - 1) * mt + k + 2], bm, flag)) return;
+        if (lane == 63) Bn[15] = bm;
       }
       // y = tau D v and u = Bn v (row sums over the half-wave)
       double q[16];
@@ -719,40 +783,43 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB
         D[i] -= vr[i] * yc + (yr[i] - K * vr[i]) * vc;
         Bn[i] -= tau * us[16 * h + i] * vc;
       }
-      // stores: L, the lower half of D, and Bn when no later task of this sweep takes it
+      // stores: L (its head from task 2 on belongs to sweep s + 1), the lower
+      // half of D, and Bn when no later task of this sweep takes it
       if (k == 0) {
         if (c == 0) {
 #pragma unroll
           for (int i = 0; i < 16; ++i) {
             const int r = 16 * h + i;
-            if (r < l) AB[(long)s * LDB + 1 + r] = L[i];
+            if (r < l) st_band(&AB[(long)s * LDB + 1 + r], L[i]);
           }
         }
       } else {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           const int r = 16 * h + i;
-          if (r < l) AB[(long)(p - B + c) * LDB + B + r - c] = L[i];
+          if (r < l && !(k >= 2 && r == 0 && c == 0))
+            st_band(&AB[(long)(p - B + c) * LDB + B + r - c], L[i]);
         }
       }
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const int r = 16 * h + i;
-        if (r < l && c < l && r >= c) AB[(long)(p + c) * LDB + (r - c)] = D[i];
+        if (r < l && c < l && r >= c) st_band(&AB[(long)(p + c) * LDB + (r - c)], D[i]);
       }
       if (k + 1 == nt && l2 > 0) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           const int r = 16 * h + i;
-          if (r < l2 && c < l) AB[(long)(p + c) * LDB + B + r - c] = Bn[i];
+          if (r < l2 && c < l) st_band(&AB[(long)(p + c) * LDB + B + r - c], Bn[i]);
         }
       }
-      __syncthreads();  // every lane's stores issued before the release
-      __threadfence();
+      // publish task k once every lane's stores have completed
+      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_s_waitcnt(0);
       if (lane == 0) __hip_atomic_store(&prog[s], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
       for (int i = 0; i < 16; ++i) L[i] = Bn[i];
-      __syncthreads();
     }
   }
 }
@@ -768,6 +835,9 @@ __global__ __launch_bounds__(256) void k_sbr_de(int n, const double *__restrict_
 
 // ---- host -----------------------------------------------------------------
 
+// tasks of sweep 0, the longest sweep (mailbox slots per sweep)
+int sbr_mail_tasks(int n) { return n >= 3 ? (n - 3) / B + 1 : 1; }
+
 SbrSizes sbr_sizes(int n) {
   SbrSizes z{};
   const int L0 = std::max(1, ceil_div(n, NT));
@@ -782,6 +852,7 @@ SbrSizes sbr_sizes(int n) {
   z.mcnt = (size_t)ceil_div(nb, 16) + 2;
   z.AB = (size_t)n * LDB;
   z.prog = (size_t)n;
+  z.mail = (size_t)n * sbr_mail_tasks(n);
   return z;
 }
 
@@ -834,15 +905,16 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
     c = r0;
     pw = B;
   }
-  hipLaunchKernelGGL(k_sbr_band, dim3(ceil_div((long)n * LDB, 256)), dim3(256), 0, s, n, A, lda,
-                     w.AB, w.prog, w.flag);
+  const SbrSizes zz = sbr_sizes(n);
+  hipLaunchKernelGGL(k_sbr_band, dim3(std::min(ceil_div((long)std::max<size_t>(zz.AB, zz.mail), 256), 8192)),
+                     dim3(256), 0, s, n, A, lda, w.AB, w.prog, w.flag, w.mail, (long)zz.mail);
   MBD_LAUNCH_CHECK();
   if (n >= 3) {
     // about n^2 / (2B) tasks, each loading two B x B blocks and storing one and
     // a half (3.5 B^2 doubles) and doing about 12 B^2 flops
     ProfScope ps(K_SBR_CHASE, s, 8.0 * 3.5 * B * 0.5 * (double)n * n, 12.0 * B * 0.5 * (double)n * n);
     hipLaunchKernelGGL(k_sbr_chase, dim3(std::min(n - 2, kSbrChaseWG)), dim3(64), 0, s, n, w.AB,
-                       w.prog, w.flag);
+                       w.prog, w.mail, sbr_mail_tasks(n), w.flag);
     MBD_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_sbr_de, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w.AB, w.d, w.e);
