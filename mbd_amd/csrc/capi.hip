@@ -200,6 +200,8 @@ struct mbd_handle {
   DevBuf<double> sbrV, sbrT, sbrR, sbrX, sbrY, sbrMp, sbrM, sbrAB;
   DevBuf<int> sbrCnt, sbrMcnt, sbrProg, sbrFlag;
   DevBuf<unsigned long long> sbrMail;
+  DevBuf<int> sbrCq;
+  DevBuf<double> sbrCqR;
   int eig_used = 1;
   int eig_flag = 0;
   // instrumentation
@@ -499,9 +501,11 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
     h->sbrProg.ensure(z.prog);
     h->sbrFlag.ensure(4);
     h->sbrMail.ensure(z.mail);
+    h->sbrCq.ensure(z.cq);
+    h->sbrCqR.ensure(z.cqR);
     sw = SbrWork{h->sbrV.p, h->sbrT.p, h->sbrR.p, h->sbrCnt.p, h->sbrX.p, h->sbrY.p,
                  h->sbrMp.p, h->sbrM.p, h->sbrMcnt.p, h->sbrAB.p, h->sbrProg.p, h->sbrFlag.p,
-                 h->sbrMail.p, h->VW.p, nm, h->dd.p, h->ee.p};
+                 h->sbrMail.p, h->sbrCq.p, h->sbrCqR.p, h->VW.p, nm, h->dd.p, h->ee.p};
   }
   auto reduce = [&]() {
     if (method == 2)
@@ -517,7 +521,8 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
                                            h->C.p, h->VW.p, h->tau.p, h->dd.p, h->ee.p, h->alph.p,
                                            h->pa.p, h->pb.p, h->py.p, h->qred.p, h->ps.p,
                                            sw.Vn, sw.Tn, sw.Rn, sw.cnt, sw.X, sw.Yp, sw.Mp, sw.M,
-                                           sw.mcnt, sw.AB, sw.prog, sw.flag, sw.mail};
+                                           sw.mcnt, sw.AB, sw.prog, sw.flag, sw.mail, sw.cq,
+                                           sw.cqR};
     if (!h->trd_exec || h->trd_key != key) {
       if (h->trd_exec) (void)hipGraphExecDestroy(h->trd_exec);
       h->trd_exec = nullptr;
@@ -1272,7 +1277,7 @@ int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w) {
       SbrWork sw{alloc(z.V), alloc(z.TR), alloc(z.TR), ialloc(z.cnt), alloc(z.X), alloc(z.Yp),
                  alloc(z.Mp), alloc((size_t)kSbrB * kSbrB), ialloc(z.mcnt), alloc(z.AB),
                  ialloc(z.prog), ialloc(4), reinterpret_cast<unsigned long long *>(alloc(z.mail)),
-                 wk.VW, wk.ldv, wk.d, wk.e};
+                 ialloc(z.cq), alloc(z.cqR), wk.VW, wk.ldv, wk.d, wk.e};
       sbr_tridiagonalize(nullptr, n, dA, n, sw);
       MBD_HIP(hipMemcpy(&abandoned, sw.flag, sizeof(int), hipMemcpyDeviceToHost));
     } else {

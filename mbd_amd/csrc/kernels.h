@@ -83,13 +83,15 @@ struct SbrWork {
   int *prog;             // tasks completed per sweep, n
   int *flag;             // [0]: bulge chase abandoned (spin limit)
   unsigned long long *mail;  // reflector heads passed between sweeps, n x sbr_mail_tasks(n)
+  int *cq;               // CholeskyQR2 panels: [0], [1] arrival counters, [2 + p] path flag of panel p
+  double *cqR;           // R1 and R2 of the current panel, 2 x B x B
   double *VW;            // [V | Z | V], n x 3B (ld ldv)
   long ldv;
   double *d, *e;         // tridiagonal output
 };
 // element counts of the work buffers for order n
 struct SbrSizes {
-  size_t nodes, V, TR, cnt, X, Yp, Mp, mcnt, AB, prog, mail;
+  size_t nodes, V, TR, cnt, X, Yp, Mp, mcnt, AB, prog, mail, cq, cqR;
 };
 SbrSizes sbr_sizes(int n);
 int sbr_mail_tasks(int n);

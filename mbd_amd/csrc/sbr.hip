@@ -116,52 +116,61 @@ __device__ inline void block_sum32(double (&q)[32], double (*part)[32], double *
 // column: with x the column below the head, sum_t x_t a_t[k] gives |x|^2
 // (k = j), the dots of the reflector with the trailing columns (k > j) and
 // with the earlier reflectors (k < j, for T).
-// Step J of node_qr, J a template parameter so that the register row a[] is
-// only ever indexed by constants.
-template <int J>
-__device__ inline void qr_step(double (&a)[32], double (*part)[32], double *S, double (*rowj)[32],
-                               double (*Ts)[B + 1]) {
-  const int t = threadIdx.x;
-  double *rj = rowj[J & 1];
-  const double x = a[J];
-  const bool below = t > J;
-  double q[32];
-#pragma unroll
-  for (int k = 0; k < 32; ++k) q[k] = below ? x * a[k] : 0.0;
-  if (t == J) {
-#pragma unroll
-    for (int k = 0; k < 32; ++k) rj[k] = a[k];
-  }
-  block_sum32(q, part, S);
-  double tau, beta, sc;
-  reflector(S[J], rj[J], tau, beta, sc);
-  if (t >= J) {
-    const double tv = tau * ((t == J) ? 1.0 : sc * x);
-#pragma unroll
-    for (int k = J + 1; k < 32; ++k) a[k] -= tv * (rj[k] + sc * S[k]);
-    a[J] = (t == J) ? beta : sc * x;
-  }
-  if (t < J) {
-    // T[t][J] = -tau sum_{l=t}^{J-1} T[t][l] (V[:, l]^T v), unrolled over l
-    double acc = 0.0;
-#pragma unroll
-    for (int l = 0; l < J; ++l) acc += (l >= t) ? Ts[t][l] * (rj[l] + sc * S[l]) : 0.0;
-    Ts[t][J] = -tau * acc;
-  } else if (t == J) {
-    Ts[J][J] = tau;
-  }
-}
-
-template <int J>
-__device__ inline void qr_steps(double (&a)[32], double (*part)[32], double *S, double (*rowj)[32],
-                                double (*Ts)[B + 1]) {
-  qr_step<J>(a, part, S, rowj, Ts);
-  if constexpr (J + 1 < B) qr_steps<J + 1>(a, part, S, rowj, Ts);
-}
-
 __device__ inline void node_qr(double (&a)[32], double (*part)[32], double *S, double (*rowj)[32],
-                               double (*Ts)[B + 1]) {
-  qr_steps<0>(a, part, S, rowj, Ts);
+                               double (*Ts)[B + 1], double *taus) {
+  const int t = threadIdx.x;
+  // The register row rotates by one slot per step, so column j is always in
+  // slot 0 and slot k holds column (j + k) mod 32: the step loop runs without
+  // unrolling and a[] is only ever indexed by constants.
+#pragma unroll 1
+  for (int j = 0; j < B; ++j) {
+    double *rj = rowj[j & 1];
+    const double x = a[0];
+    const double xb = (t > j) ? x : 0.0;
+    double q[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) q[k] = xb * a[k];
+    if (t == j) {
+#pragma unroll
+      for (int k = 0; k < 32; ++k) rj[k] = a[k];
+    }
+    block_sum32(q, part, S);
+    double tau, beta, sc;
+    reflector(S[0], rj[0], tau, beta, sc);
+    const int ntr = B - 1 - j;  // trailing columns j+1 .. 31 sit in slots 1 .. ntr
+    // branch-free: every slot reads its dot first (the LDS loads issue together),
+    // slots of finished columns get a zero factor
+    const double tv = (t >= j) ? tau * ((t == j) ? 1.0 : sc * x) : 0.0;
+    double w[32];
+#pragma unroll
+    for (int k = 1; k < 32; ++k) w[k] = rj[k] + sc * S[k];
+#pragma unroll
+    for (int k = 1; k < 32; ++k) a[k] = fma(-((k <= ntr) ? tv : 0.0), w[k], a[k]);
+    if (t >= j) a[0] = (t == j) ? beta : sc * x;
+    // Gram column V[:, t]^T v_j (column t < j sits in slot t - j + 32), kept for T
+    if (t < j) Ts[t][j] = rj[t - j + 32] + sc * S[t - j + 32];
+    if (t == j) taus[j] = tau;
+    const double a0 = a[0];
+#pragma unroll
+    for (int k = 0; k < 31; ++k) a[k] = a[k + 1];
+    a[31] = a0;
+  }
+  __syncthreads();
+  // T of H_0 ... H_31 = I - V T V^T from the Gram columns, column by column:
+  // T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j], T[j][j] = tau_j (the strict upper
+  // part of Ts holds G on entry and T on exit; thread i owns row i)
+  for (int j = 0; j < B; ++j) {
+    double acc = 0.0;
+    if (t < j) {
+#pragma unroll
+      for (int l = 0; l < B; ++l)
+        if (l >= t && l < j) acc += Ts[t][l] * Ts[l][j];
+    }
+    __syncthreads();  // column j is read as G by every row before any row is replaced
+    if (t < j) Ts[t][j] = -taus[j] * acc;
+    if (t == j) Ts[j][j] = taus[j];
+    __syncthreads();
+  }
 }
 
 // Reflector element (r, i) of a node stored as in node_qr (unit diagonal,
@@ -180,11 +189,13 @@ __device__ inline double vget(const double *V, int r, int i) {
 __global__ __launch_bounds__(NT) void k_sbr_tsqr(int m, int w, const double *__restrict__ P,
                                                  long lda, double *__restrict__ Vn,
                                                  double *__restrict__ Tn, double *__restrict__ Rn,
-                                                 int *cnt, Tree tr) {
+                                                 int *cnt, Tree tr, const int *use) {
+  if (*use == 0) return;  // the CholeskyQR2 path factored this panel
   __shared__ double part[NW][32];
   __shared__ double S[32];
   __shared__ double rowj[2][32];
   __shared__ double Ts[B][B + 1];
+  __shared__ double taus[B];
   __shared__ int last;
   const int t = threadIdx.x;
   int lev = 0, idx = blockIdx.x;
@@ -196,7 +207,7 @@ __global__ __launch_bounds__(NT) void k_sbr_tsqr(int m, int w, const double *__r
   for (;;) {
     for (int e = t; e < B * (B + 1); e += NT) (&Ts[0][0])[e] = 0.0;
     __syncthreads();
-    node_qr(a, part, S, rowj, Ts);
+    node_qr(a, part, S, rowj, Ts, taus);
     __syncthreads();
     const long node = tr.off[lev] + idx;
     double *V = Vn + node * NT * B;
@@ -237,7 +248,9 @@ __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__
                                                   const double *__restrict__ Rn, Tree tr,
                                                   double *__restrict__ VW, long ldv,
                                                   double *__restrict__ X, long ldx,
-                                                  double *__restrict__ Ab, long lda) {
+                                                  double *__restrict__ Ab, long lda,
+                                                  const int *use) {
+  if (*use == 0) return;
   __shared__ double Qs[B][B + 1];  // Q block entering the current node
   __shared__ double M1[B][B + 1];
   __shared__ double M2[B][B + 1];
@@ -330,24 +343,33 @@ __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__
     W[r][k] = W[r][k] * sgn[k] - (r == k ? 1.0 : 0.0);
   }
   __syncthreads();
-  // LU without pivoting (|pivot| >= 1 by the choice of signs)
+  // LU without pivoting (|pivot| >= 1 by the choice of signs); the multipliers
+  // of step kk go to column kk of W only after the step's updates have read it
   for (int kk = 0; kk < B - 1; ++kk) {
-    if (t > kk && t < B) W[t][kk] /= W[kk][kk];
+    const double rpiv = 1.0 / W[kk][kk];
+    double lm[B * B / NT];
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q, r = e / B, c = e % B;
+      lm[q] = W[r][kk] * rpiv;
+      if (r > kk && c > kk) W[r][c] -= lm[q] * W[kk][c];
+    }
     __syncthreads();
-    for (int e = t; e < B * B; e += NT) {
-      const int r = e / B, c = e % B;
-      if (r > kk && c > kk) W[r][c] -= W[r][kk] * W[kk][c];
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q, r = e / B, c = e % B;
+      if (r > kk && c == kk) W[r][c] = lm[q];
     }
     __syncthreads();
   }
   // Tf = -U Y1^{-T}: columns of Z = Y1^{-T} (unit upper) into M1
   if (t < B) {
     const int k = t;
-    for (int i = B - 1; i > k; --i) M1[i][k] = 0.0;
-    M1[k][k] = 1.0;
+    for (int i = 0; i < B; ++i) M1[i][k] = (i == k) ? 1.0 : 0.0;
     for (int i = k - 1; i >= 0; --i) {
       double s = 0.0;
-      for (int l = i + 1; l <= k; ++l) s += W[l][i] * M1[l][k];
+#pragma unroll
+      for (int l = 0; l < B; ++l) s += (l > i && l <= k) ? W[l][i] * M1[l][k] : 0.0;
       M1[i][k] = -s;
     }
   }
@@ -355,7 +377,8 @@ __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__
   for (int e = t; e < B * B; e += NT) {
     const int i = e / B, k = e % B;
     double s = 0.0;
-    for (int l = i; l <= k; ++l) s += W[i][l] * M1[l][k];
+#pragma unroll
+    for (int l = 0; l < B; ++l) s += (l >= i && l <= k) ? W[i][l] * M1[l][k] : 0.0;
     Tf[i][k] = (i <= k) ? -s : 0.0;
   }
   __syncthreads();
@@ -381,16 +404,20 @@ __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__
         for (int i = 0; i < B; ++i) s -= vrow[i] * M2[i][k];
         y[k] = s * sgn[k];
       }
-      // y U = q S in place (U upper: W on and above the diagonal)
+      // y U = q S in place (U upper: W on and above the diagonal); inner sums
+      // over all B terms with a zero factor past the triangle, so the LDS loads
+      // issue together instead of one round trip per term
       for (int k = 0; k < B; ++k) {
         double s = y[k];
-        for (int i = 0; i < k; ++i) s -= y[i] * W[i][k];
+#pragma unroll
+        for (int i = 0; i < B; ++i) s -= (i < k) ? y[i] * W[i][k] : 0.0;
         y[k] = s / W[k][k];
       }
     }
     for (int k = 0; k < B; ++k) {
       double s = 0.0;
-      for (int i = 0; i <= k; ++i) s += y[i] * Tf[i][k];
+#pragma unroll
+      for (int i = 0; i < B; ++i) s += (i <= k) ? y[i] * Tf[i][k] : 0.0;
       VW[g + (long)k * ldv] = y[k];
       VW[g + (long)(2 * B + k) * ldv] = y[k];
       X[g + (long)k * ldx] = s;
@@ -399,6 +426,305 @@ __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__
   if (leaf == 0 && t < B) {
     const double *R = Rn + (long)tr.off[tr.nlev - 1] * B * B;
     for (int k = 0; k < w; ++k) Ab[t + (long)k * lda] = (k >= t) ? sgn[t] * R[t * B + k] : 0.0;
+  }
+}
+
+// ---- CholeskyQR2 panel path -----------------------------------------------
+// P = Q1 R1 with R1 = chol(P^T P), then Q = Q1 R2^{-1} with R2 = chol(Q1^T Q1)
+// (orthogonal to working precision for cond(P) up to ~1e7; the panels of the
+// coupled-QHO matrix measure cond < 1e2).  A panel whose Cholesky pivot is not
+// clearly positive or whose Q1^T Q1 is far from I sets its flag and is
+// factored by the TSQR kernels instead (zero or rank-deficient panels, and
+// the narrower first panel, always take that path).
+namespace {
+typedef double Mat32[B + 1];
+
+// C = alpha A Bm, all 32 x 32 in LDS with explicit zeros (C distinct from A, Bm)
+__device__ inline void lds_mm(Mat32 *C, const Mat32 *A, const Mat32 *Bm, double alpha) {
+  for (int e = threadIdx.x; e < B * B; e += NT) {
+    const int i = e / B, k = e % B;
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < B; ++l) s += A[i][l] * Bm[l][k];
+    C[i][k] = alpha * s;
+  }
+  __syncthreads();
+}
+
+// Ai = A^{-1} for A upper triangular (zeros below).  Thread c < B computes
+// column c in registers, fully unrolled from the bottom row up (static
+// indices, so the loads of A issue ahead of the dependent chain).
+__device__ inline void lds_inv_upper(const Mat32 *A, Mat32 *Ai) {
+  const int t = threadIdx.x;
+  if (t < B) {
+    double col[B], rd[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) rd[i] = 1.0 / A[i][i];
+#pragma unroll
+    for (int i = B - 1; i >= 0; --i) {
+      double s0 = (i == t) ? 1.0 : 0.0, s1 = 0.0;
+#pragma unroll
+      for (int l = i + 1; l < B; ++l) {
+        if (l & 1)
+          s1 -= A[i][l] * col[l];
+        else
+          s0 -= A[i][l] * col[l];
+      }
+      col[i] = (s0 + s1) * rd[i];
+    }
+#pragma unroll
+    for (int i = 0; i < B; ++i) Ai[i][t] = col[i];
+  }
+  __syncthreads();
+}
+
+// Ai = A^{-1} for A unit lower triangular (strict lower part stored, zeros
+// elsewhere); same organisation, top row down.
+__device__ inline void lds_inv_unit_lower(const Mat32 *A, Mat32 *Ai) {
+  const int t = threadIdx.x;
+  if (t < B) {
+    double col[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      double s0 = (i == t) ? 1.0 : 0.0, s1 = 0.0;
+#pragma unroll
+      for (int l = 0; l < i; ++l) {
+        if (l & 1)
+          s1 -= A[i][l] * col[l];
+        else
+          s0 -= A[i][l] * col[l];
+      }
+      col[i] = s0 + s1;
+    }
+#pragma unroll
+    for (int i = 0; i < B; ++i) Ai[i][t] = col[i];
+  }
+  __syncthreads();
+}
+
+// Upper Cholesky G = R^T R: R (zeros below) from the upper triangle of G, which
+// is overwritten.  False when a pivot is not above tol.  One barrier per step:
+// the trailing update uses the unscaled row k divided by its pivot.
+__device__ inline bool lds_chol32(Mat32 *G, Mat32 *R, double tol) {
+  const int t = threadIdx.x;
+  for (int e = t; e < B * B; e += NT) R[e / B][e % B] = 0.0;
+  __syncthreads();
+  bool ok = true;
+  for (int k = 0; k < B; ++k) {
+    const double d = G[k][k];
+    ok = ok && (d > tol);
+    const double dd = (d > tol) ? d : 1.0;
+    const double rs = 1.0 / sqrt(dd), rdd = 1.0 / dd;
+    double nv[B * B / NT];
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q, i = e / B, j = e % B;
+      nv[q] = (i > k && j >= i) ? G[i][j] - G[k][i] * G[k][j] * rdd : 0.0;
+      if (i == k && j >= k) R[k][j] = G[k][j] * rs;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q, i = e / B, j = e % B;
+      if (i > k && j >= i) G[i][j] = nv[q];
+    }
+    __syncthreads();
+  }
+  return ok;
+}
+
+// Partial Gram Ps^T Ps of this block's rows (Ps[col][row], NT rows) on MFMA:
+// wave v owns the 16 x 16 tile (v >> 1, v & 1); written to Gp.
+__device__ inline void block_gram(const double (*Ps)[NT + 1], double *Gp) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int ti = wv >> 1, tj = wv & 1, lr = lane & 15, lk = lane >> 4;
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int k4 = 0; k4 < NT / 4; ++k4) {
+    const double av = Ps[16 * ti + lr][4 * k4 + lk];
+    const double bv = Ps[16 * tj + lr][4 * k4 + lk];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Gp[(16 * ti + lk + 4 * v) * B + 16 * tj + lr] = acc[v];
+}
+
+// Last of the grid's blocks to arrive sums the partial Grams into G (LDS).
+__device__ inline bool gram_reduce_last(const double *Gp, int *cnt, Mat32 *G) {
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = (atomicAdd(cnt, 1) == (int)gridDim.x - 1);
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  if (threadIdx.x == 0) *cnt = 0;
+  for (int e = threadIdx.x; e < B * B; e += NT) {
+    double s = 0.0;
+    for (int b = 0; b < (int)gridDim.x; ++b) s += Gp[(long)b * B * B + e];
+    G[e / B][e % B] = s;
+  }
+  __syncthreads();
+  return true;
+}
+}  // namespace
+
+// R1 = chol(P^T P) into Rbuf[0 .. B*B); *fail = 1 if the Cholesky is not clean.
+__global__ __launch_bounds__(NT) void k_sbr_cq1(int m, int w, const double *__restrict__ P, long lda,
+                                                double *__restrict__ Gp, int *cnt,
+                                                double *__restrict__ Rbuf, int *fail) {
+  __shared__ double Ps[B][NT + 1];
+  __shared__ Mat32 G[B], R[B];
+  const int t = threadIdx.x, row = NT * blockIdx.x + t;
+#pragma unroll
+  for (int k = 0; k < B; ++k) Ps[k][t] = (row < m && k < w) ? P[row + (long)k * lda] : 0.0;
+  __syncthreads();
+  block_gram(Ps, Gp + (long)blockIdx.x * B * B);
+  if (!gram_reduce_last(Gp, cnt, G)) return;
+  double dmax = 0.0;
+  for (int k = 0; k < B; ++k) dmax = fmax(dmax, G[k][k]);
+  const bool ok = lds_chol32(G, R, 1e-10 * dmax) && dmax > 0.0;
+  for (int e = t; e < B * B; e += NT) Rbuf[e] = R[e / B][e % B];
+  if (t == 0) *fail = ok ? 0 : 1;
+}
+
+// Q1 = P R1^{-1} (rows into Q1), R2 = chol(Q1^T Q1) into Rbuf[B*B .. 2 B*B);
+// *fail = 1 if Q1^T Q1 is not close to I or its Cholesky is not clean.
+__global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__restrict__ P, long lda,
+                                                double *__restrict__ Q1, long ldq,
+                                                double *__restrict__ Gp, int *cnt,
+                                                double *__restrict__ Rbuf, int *fail) {
+  if (*fail) return;
+  __shared__ double Ps[B][NT + 1];
+  __shared__ Mat32 R[B], Ri[B], G[B];
+  const int t = threadIdx.x, row = NT * blockIdx.x + t;
+  for (int e = t; e < B * B; e += NT) R[e / B][e % B] = Rbuf[e];
+  __syncthreads();
+  lds_inv_upper(R, Ri);
+  double pr[B];
+#pragma unroll
+  for (int k = 0; k < B; ++k) pr[k] = (row < m && k < w) ? P[row + (long)k * lda] : 0.0;
+  for (int k = 0; k < B; ++k) {
+    double q = 0.0;
+#pragma unroll
+    for (int i = 0; i < B; ++i) q += pr[i] * Ri[i][k];
+    Ps[k][t] = q;
+    if (row < m) Q1[row + (long)k * ldq] = q;
+  }
+  __syncthreads();
+  block_gram(Ps, Gp + (long)blockIdx.x * B * B);
+  if (!gram_reduce_last(Gp, cnt, G)) return;
+  double dev = 0.0;
+  for (int e = 0; e < B * B; ++e) dev = fmax(dev, fabs(G[e / B][e % B] - (e / B == e % B ? 1.0 : 0.0)));
+  const bool ok = (dev < 0.1) && lds_chol32(G, R, 0.5);
+  for (int e = t; e < B * B; e += NT) Rbuf[B * B + e] = R[e / B][e % B];
+  if (t == 0) *fail = ok ? 0 : 1;
+}
+
+// Q = Q1 R2^{-1}, Householder reconstruction of Q as in k_sbr_qform, every
+// per-row step folded into two B x B matrices: y = q1 My, x = y T = q1 Mx with
+// My = R2^{-1} S U^{-1}, Mx = My T (rows of the top block: y = Y1).  Writes V
+// (VW columns 0..B-1 and 2B..3B-1), X = V T and, from block 0, S R2 R1.
+__global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict__ Q1, long ldq,
+                                                const double *__restrict__ Rbuf, const int *fail,
+                                                double *__restrict__ VW, long ldv,
+                                                double *__restrict__ X, long ldx,
+                                                double *__restrict__ Ab, long lda) {
+  if (*fail) return;
+  __shared__ Mat32 R2[B], R2i[B], Wm[B], U[B], Y1[B], Ui[B], Yi[B], Tf[B], My[B], Mx[B];
+  __shared__ double sgn[B];
+  const int t = threadIdx.x;
+  for (int e = t; e < B * B; e += NT) {
+    R2[e / B][e % B] = Rbuf[B * B + e];
+    Y1[e / B][e % B] = Q1[e / B + (long)(e % B) * ldq];  // top block of Q1
+  }
+  __syncthreads();
+  lds_inv_upper(R2, R2i);
+  lds_mm(Wm, Y1, R2i, 1.0);  // top block of Q
+  if (t < B) sgn[t] = (Wm[t][t] >= 0.0) ? -1.0 : 1.0;
+  __syncthreads();
+  for (int e = t; e < B * B; e += NT) {
+    const int r = e / B, k = e % B;
+    Wm[r][k] = Wm[r][k] * sgn[k] - (r == k ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  // LU without pivoting (|pivot| >= 1 by the choice of signs)
+  for (int kk = 0; kk < B - 1; ++kk) {
+    const double rpiv = 1.0 / Wm[kk][kk];
+    double lm[B * B / NT];
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q, r = e / B, c = e % B;
+      lm[q] = Wm[r][kk] * rpiv;
+      if (r > kk && c > kk) Wm[r][c] -= lm[q] * Wm[kk][c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q, r = e / B, c = e % B;
+      if (r > kk && c == kk) Wm[r][c] = lm[q];
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < B * B; e += NT) {
+    const int r = e / B, k = e % B;
+    U[r][k] = (k >= r) ? Wm[r][k] : 0.0;
+    Y1[r][k] = (k < r) ? Wm[r][k] : 0.0;
+  }
+  __syncthreads();
+  lds_inv_upper(U, Ui);
+  lds_inv_unit_lower(Y1, Yi);
+  // Tf = -U Y1^{-T}
+  for (int e = t; e < B * B; e += NT) {
+    const int i = e / B, k = e % B;
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < B; ++l) s += U[i][l] * Yi[k][l];
+    Tf[i][k] = -s;
+  }
+  // My = R2^{-1} S U^{-1}: scale the rows of U^{-1} by S in place first
+  for (int e = t; e < B * B; e += NT) Ui[e / B][e % B] *= sgn[e / B];
+  __syncthreads();
+  lds_mm(My, R2i, Ui, 1.0);
+  lds_mm(Mx, My, Tf, 1.0);
+  const int g = NT * blockIdx.x + t;
+  if (g < m) {
+    double qr[B];
+    if (g < B) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) qr[k] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
+      for (int k = 0; k < B; ++k) {
+        double x = 0.0;
+#pragma unroll
+        for (int i = 0; i < B; ++i) x += qr[i] * Tf[i][k];
+        VW[g + (long)k * ldv] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
+        VW[g + (long)(2 * B + k) * ldv] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
+        X[g + (long)k * ldx] = x;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < B; ++k) qr[k] = Q1[g + (long)k * ldq];
+      for (int k = 0; k < B; ++k) {
+        double y = 0.0, x = 0.0;
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+          y += qr[i] * My[i][k];
+          x += qr[i] * Mx[i][k];
+        }
+        VW[g + (long)k * ldv] = y;
+        VW[g + (long)(2 * B + k) * ldv] = y;
+        X[g + (long)k * ldx] = x;
+      }
+    }
+  }
+  if (blockIdx.x == 0) {
+    // band block S R, R = R2 R1 (Wm is free again)
+    __syncthreads();
+    for (int e = t; e < B * B; e += NT) Y1[e / B][e % B] = Rbuf[e];
+    __syncthreads();
+    lds_mm(Wm, R2, Y1, 1.0);
+    if (t < B)
+      for (int k = 0; k < B; ++k) Ab[t + (long)k * lda] = (k >= t) ? sgn[t] * Wm[t][k] : 0.0;
   }
 }
 
@@ -570,10 +896,12 @@ __global__ __launch_bounds__(256) void k_sbr_z(int m, const double *__restrict__
   }
 }
 
-__global__ __launch_bounds__(256) void k_sbr_zero(int ncnt, int *cnt, int nm, int *mcnt) {
+__global__ __launch_bounds__(256) void k_sbr_zero(int ncnt, int *cnt, int nm, int *mcnt, int ncq,
+                                                  int *cq) {
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e < ncnt) cnt[e] = 0;
   if (e < nm) mcnt[e] = 0;
+  if (e < ncq) cq[e] = 0;
 }
 
 // ---- stage 2 --------------------------------------------------------------
@@ -756,18 +1084,18 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB
         if (!mail_wait(&mail[(long)(s - 1) * mt + k + 2], bm, flag)) return;
         if (lane == 63) Bn[15] = bm;
       }
-      // y = tau D v and u = Bn v (row sums over the half-wave)
-      double q[16];
+      // y = tau D v as column sums (D is exactly symmetric in registers: both
+      // triangles come from the same stored elements), u = Bn v as row sums
+      double yp = 0.0;
 #pragma unroll
-      for (int i = 0; i < 16; ++i) q[i] = D[i] * vc;
-      const double ysum = half_sum16(q, lane);
+      for (int i = 0; i < 16; ++i) yp += D[i] * vr[i];
+      const double ycol = yp + __shfl_xor(yp, 32, 64);
+      double q[16];
 #pragma unroll
       for (int i = 0; i < 16; ++i) q[i] = Bn[i] * vc;
       const double usum = half_sum16(q, lane);
-      if ((c & 1) == 0) {
-        ys[16 * h + (c >> 1)] = tau * ysum;
-        us[16 * h + (c >> 1)] = usum;
-      }
+      if (h == 0) ys[c] = tau * ycol;
+      if ((c & 1) == 0) us[16 * h + (c >> 1)] = usum;
       __syncthreads();
       double yr[16];
       double kp = 0.0;
@@ -853,6 +1181,8 @@ SbrSizes sbr_sizes(int n) {
   z.AB = (size_t)n * LDB;
   z.prog = (size_t)n;
   z.mail = (size_t)n * sbr_mail_tasks(n);
+  z.cq = (size_t)n / B + 4;  // two arrival counters, then one path flag per panel
+  z.cqR = 2 * (size_t)B * B;
   return z;
 }
 
@@ -864,20 +1194,31 @@ int eig_method(int n) {
 
 void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
   const SbrSizes z = sbr_sizes(n);
-  hipLaunchKernelGGL(k_sbr_zero, dim3(ceil_div((long)std::max(z.cnt, z.mcnt), 256)), dim3(256),
-                     0, s, (int)z.cnt, w.cnt, (int)z.mcnt, w.mcnt);
+  hipLaunchKernelGGL(k_sbr_zero, dim3(ceil_div((long)std::max(std::max(z.cnt, z.mcnt), z.cq), 256)),
+                     dim3(256), 0, s, (int)z.cnt, w.cnt, (int)z.mcnt, w.mcnt, (int)z.cq, w.cq);
   MBD_LAUNCH_CHECK();
-  int c = 0, pw = (n % B) ? n % B : B;
+  int c = 0, pw = (n % B) ? n % B : B, panel = 0;
   while (c + pw < n) {
     const int r0 = c + pw, m = n - r0;
     const Tree tr = make_tree(m);
+    int *fail = w.cq + 2 + panel;  // 0: CholeskyQR2 factored the panel, 1: TSQR does
     {
       ProfScope ps(K_SBR_QR, s, 8.0 * 2.0 * m * B, 4.0 * (double)m * B * B);
-      hipLaunchKernelGGL(k_sbr_tsqr, dim3(tr.L[0]), dim3(NT), 0, s, m, pw,
-                         A + r0 + (long)c * lda, lda, w.Vn, w.Tn, w.Rn, w.cnt, tr);
+      double *P = A + r0 + (long)c * lda;
+      hipLaunchKernelGGL(k_sbr_cq1, dim3(tr.L[0]), dim3(NT), 0, s, m, pw, P, lda, w.Mp, w.cq,
+                         w.cqR, fail);
+      MBD_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_sbr_cq2, dim3(tr.L[0]), dim3(NT), 0, s, m, pw, P, lda, w.Yp, (long)n,
+                         w.Mp, w.cq + 1, w.cqR, fail);
+      MBD_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_sbr_cq3, dim3(tr.L[0]), dim3(NT), 0, s, m, w.Yp, (long)n, w.cqR, fail,
+                         w.VW, w.ldv, w.X, (long)n, P, lda);
+      MBD_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_sbr_tsqr, dim3(tr.L[0]), dim3(NT), 0, s, m, pw, P, lda, w.Vn, w.Tn,
+                         w.Rn, w.cnt, tr, fail);
       MBD_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_sbr_qform, dim3(tr.L[0]), dim3(NT), 0, s, m, pw, w.Vn, w.Tn, w.Rn,
-                         tr, w.VW, w.ldv, w.X, (long)n, A + r0 + (long)c * lda, lda);
+                         tr, w.VW, w.ldv, w.X, (long)n, P, lda, fail);
       MBD_LAUNCH_CHECK();
     }
     const int nb = ceil_div(m, 64);
@@ -904,6 +1245,7 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
     }
     c = r0;
     pw = B;
+    ++panel;
   }
   const SbrSizes zz = sbr_sizes(n);
   hipLaunchKernelGGL(k_sbr_band, dim3(std::min(ceil_div((long)std::max<size_t>(zz.AB, zz.mail), 256), 8192)),
