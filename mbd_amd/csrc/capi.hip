@@ -494,7 +494,7 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
     h->sbrX.ensure(z.X);
     h->sbrY.ensure(z.Yp);
     h->sbrMp.ensure(z.Mp);
-    h->sbrM.ensure((size_t)kSbrB * kSbrB);
+    h->sbrM.ensure((size_t)3 * kSbrB * kSbrB);
     h->sbrAB.ensure(z.AB);
     h->sbrCnt.ensure(z.cnt);
     h->sbrMcnt.ensure(z.mcnt);
@@ -1275,7 +1275,7 @@ int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w) {
         return p;
       };
       SbrWork sw{alloc(z.V), alloc(z.TR), alloc(z.TR), ialloc(z.cnt), alloc(z.X), alloc(z.Yp),
-                 alloc(z.Mp), alloc((size_t)kSbrB * kSbrB), ialloc(z.mcnt), alloc(z.AB),
+                 alloc(z.Mp), alloc((size_t)3 * kSbrB * kSbrB), ialloc(z.mcnt), alloc(z.AB),
                  ialloc(z.prog), ialloc(4), reinterpret_cast<unsigned long long *>(alloc(z.mail)),
                  ialloc(z.cq), alloc(z.cqR), wk.VW, wk.ldv, wk.d, wk.e};
       sbr_tridiagonalize(nullptr, n, dA, n, sw);

@@ -77,7 +77,7 @@ struct SbrWork {
   double *X;             // V T, n x B (ld n)
   double *Yp;            // split partials of A22 X, kSbrMaxSplit x B x n
   double *Mp;            // partials of X^T A22 X
-  double *M;             // X^T A22 X, B x B
+  double *M;             // X^T A22 X and the pending-update products, 3 x B x B
   int *mcnt;             // arrival counters of the M reduction
   double *AB;            // band, kSbrLdb x n (column-major)
   int *prog;             // tasks completed per sweep, n
@@ -85,7 +85,7 @@ struct SbrWork {
   unsigned long long *mail;  // reflector heads passed between sweeps, n x sbr_mail_tasks(n)
   int *cq;               // CholeskyQR2 panels: [0], [1] arrival counters, [2 + p] path flag of panel p
   double *cqR;           // R1 and R2 of the current panel, 2 x B x B
-  double *VW;            // [V | Z | V], n x 3B (ld ldv)
+  double *VW;            // two panels' factors [V_a | V_b | Z_a | Z_b | V_a | V_b], n x 6B (ld ldv)
   long ldv;
   double *d, *e;         // tridiagonal output
 };

@@ -241,13 +241,13 @@ __global__ __launch_bounds__(NT) void k_sbr_tsqr(int m, int w, const double *__r
 }
 
 // Explicit Q of the panel (rows of this workgroup's leaf), Householder
-// reconstruction and the outputs V (into VW columns 0..B-1 and 2B..3B-1),
+// reconstruction and the outputs V (into V1 and V2),
 // X = V T and, from workgroup 0, the band block S R (w columns) at Ab.
 __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__restrict__ Vn,
                                                   const double *__restrict__ Tn,
                                                   const double *__restrict__ Rn, Tree tr,
-                                                  double *__restrict__ VW, long ldv,
-                                                  double *__restrict__ X, long ldx,
+                                                  double *__restrict__ V1, double *__restrict__ V2,
+                                                  long ldv, double *__restrict__ X, long ldx,
                                                   double *__restrict__ Ab, long lda,
                                                   const int *use) {
   if (*use == 0) return;
@@ -418,8 +418,8 @@ __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__
       double s = 0.0;
 #pragma unroll
       for (int i = 0; i < B; ++i) s += (i <= k) ? y[i] * Tf[i][k] : 0.0;
-      VW[g + (long)k * ldv] = y[k];
-      VW[g + (long)(2 * B + k) * ldv] = y[k];
+      V1[g + (long)k * ldv] = y[k];
+      V2[g + (long)k * ldv] = y[k];
       X[g + (long)k * ldx] = s;
     }
   }
@@ -624,11 +624,11 @@ __global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__re
 // Q = Q1 R2^{-1}, Householder reconstruction of Q as in k_sbr_qform, every
 // per-row step folded into two B x B matrices: y = q1 My, x = y T = q1 Mx with
 // My = R2^{-1} S U^{-1}, Mx = My T (rows of the top block: y = Y1).  Writes V
-// (VW columns 0..B-1 and 2B..3B-1), X = V T and, from block 0, S R2 R1.
+// (into V1 and V2), X = V T and, from block 0, S R2 R1.
 __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict__ Q1, long ldq,
                                                 const double *__restrict__ Rbuf, const int *fail,
-                                                double *__restrict__ VW, long ldv,
-                                                double *__restrict__ X, long ldx,
+                                                double *__restrict__ V1, double *__restrict__ V2,
+                                                long ldv, double *__restrict__ X, long ldx,
                                                 double *__restrict__ Ab, long lda) {
   if (*fail) return;
   __shared__ Mat32 R2[B], R2i[B], Wm[B], U[B], Y1[B], Ui[B], Yi[B], Tf[B], My[B], Mx[B];
@@ -697,8 +697,8 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
         double x = 0.0;
 #pragma unroll
         for (int i = 0; i < B; ++i) x += qr[i] * Tf[i][k];
-        VW[g + (long)k * ldv] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
-        VW[g + (long)(2 * B + k) * ldv] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
+        V1[g + (long)k * ldv] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
+        V2[g + (long)k * ldv] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
         X[g + (long)k * ldx] = x;
       }
     } else {
@@ -711,8 +711,8 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
           y += qr[i] * My[i][k];
           x += qr[i] * Mx[i][k];
         }
-        VW[g + (long)k * ldv] = y;
-        VW[g + (long)(2 * B + k) * ldv] = y;
+        V1[g + (long)k * ldv] = y;
+        V2[g + (long)k * ldv] = y;
         X[g + (long)k * ldx] = x;
       }
     }
@@ -811,17 +811,24 @@ __global__ __launch_bounds__(256) void k_sbr_symm(int m, const double *__restric
 
 // Y = sum of the split partials (written over split 0), and M = X^T Y: each
 // 64-row block's product goes to Mp, groups of 16 blocks are summed by their
-// last arrival, the group sums by the last group.
+// last arrival, the group sums by the last group.  With a pending update
+// A22 -= Vq Zq^T + Zq Vq^T not yet applied to A22 (Vq, Zq non-null), the block
+// products GV = Vq^T X and GZ = Zq^T X ride along and the true
+// M = X^T Y - GV^T GZ - GZ^T GV; M, GV, GZ are stored at M[0], M[B*B], M[2*B*B].
 __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__restrict__ Yp,
                                                   long ldy, const double *__restrict__ X,
-                                                  long ldx, double *__restrict__ Mp, int *mcnt,
+                                                  long ldx, const double *__restrict__ Vq,
+                                                  const double *__restrict__ Zq, long ldq,
+                                                  double *__restrict__ Mp, int *mcnt,
                                                   double *__restrict__ M) {
-  constexpr int G = 16;
-  __shared__ double Ys[64][B + 1], Xs[64][B + 1];
+  constexpr int G = 16, NQ = 3 * B * B;
+  // column-major staging: Ys[c][r], Xs[c][r] (r = row within the 64-row block)
+  __shared__ double Ys[B][64 + 1], Xs[B][64 + 1];
   __shared__ int last;
   const int t = threadIdx.x, I = blockIdx.x, nb = gridDim.x;
   const int r = t & 63, cg = t >> 6;
   const int row = 64 * I + r;
+  const bool pend = Vq != nullptr;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     const int c = 8 * cg + j;
@@ -831,17 +838,37 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
       Yp[row + (long)c * ldy] = s;
       xv = X[row + (long)c * ldx];
     }
-    Ys[r][c] = s;
-    Xs[r][c] = xv;
+    Ys[c][r] = s;
+    Xs[c][r] = xv;
   }
   __syncthreads();
+  // 32 x 32 products over the block's 64 rows on MFMA: wave v owns tile (v >> 1, v & 1)
+  const int lane = t & 63, ti = cg >> 1, tj = cg & 1, lr = lane & 15, lk = lane >> 4;
+  auto tile = [&](const double (*L)[64 + 1], const double (*R)[64 + 1], double *out) {
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int e = t + 256 * i, a = e >> 5, b = e & 31;
-    double s = 0.0;
-    for (int rr = 0; rr < 64; ++rr) s += Xs[rr][a] * Ys[rr][b];
-    Mp[(long)I * B * B + e] = s;
+    for (int k4 = 0; k4 < 16; ++k4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(L[16 * ti + lr][4 * k4 + lk], R[16 * tj + lr][4 * k4 + lk],
+                                                acc, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[(16 * ti + lk + 4 * v) * B + 16 * tj + lr] = acc[v];
+  };
+  tile(Xs, Ys, Mp + (long)I * NQ);
+  if (pend) {
+    // reuse Ys for the pending V block, then the pending Z block
+    for (int pass = 0; pass < 2; ++pass) {
+      const double *Q = pass ? Zq : Vq;
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int c = 8 * cg + j;
+        Ys[c][r] = (row < m) ? Q[row + (long)c * ldq] : 0.0;
+      }
+      __syncthreads();
+      tile(Ys, Xs, Mp + (long)I * NQ + (1 + pass) * B * B);
+    }
   }
+  const int nq = pend ? NQ : B * B;
   __threadfence();
   __syncthreads();
   const int grp = I / G, ng = (nb + G - 1) / G, gsz = min(G, nb - grp * G);
@@ -850,13 +877,11 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
   if (!last) return;
   __threadfence();
   if (t == 0) mcnt[1 + grp] = 0;
-  double *Mg = Mp + (long)nb * B * B;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int e = t + 256 * i;
+  double *Mg = Mp + (long)nb * NQ;
+  for (int e = t; e < nq; e += 256) {
     double s = 0.0;
-    for (int q = 0; q < gsz; ++q) s += Mp[(long)(grp * G + q) * B * B + e];
-    Mg[(long)grp * B * B + e] = s;
+    for (int q = 0; q < gsz; ++q) s += Mp[(long)(grp * G + q) * NQ + e];
+    Mg[(long)grp * NQ + e] = s;
   }
   __threadfence();
   __syncthreads();
@@ -865,34 +890,78 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
   if (!last) return;
   __threadfence();
   if (t == 0) mcnt[0] = 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int e = t + 256 * i;
+  // GV and GZ also go to LDS (Ys, Xs are free) for the correction of M
+  double *gv = &Ys[0][0], *gz = &Xs[0][0];
+  for (int e = t; e < nq; e += 256) {
     double s = 0.0;
-    for (int g = 0; g < ng; ++g) s += Mg[(long)g * B * B + e];
-    M[e] = s;
+    for (int g = 0; g < ng; ++g) s += Mg[(long)g * NQ + e];
+    if (e >= 2 * B * B)
+      gz[e - 2 * B * B] = s;
+    else if (e >= B * B)
+      gv[e - B * B] = s;
+    if (!pend || e >= B * B) M[e] = s;
+    if (pend && e < B * B) Mp[e] = s;  // uncorrected X^T Y, in a partial slot already consumed
+  }
+  if (pend) {
+    __syncthreads();
+    // M = X^T Y - GV^T GZ - GZ^T GV (row-major 32 x 32 blocks)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = t + 256 * i, a = e >> 5, b = e & 31;
+      double s = 0.0;
+#pragma unroll 8
+      for (int l = 0; l < B; ++l) s += gv[l * B + a] * gz[l * B + b] + gz[l * B + a] * gv[l * B + b];
+      M[e] = Mp[e] - s;
+    }
   }
 }
 
-// Z = Y - V M / 2 into VW columns B..2B-1 (one thread per row).
+// Z = Y - V M / 2, and with a pending update Z -= Vq GZ + Zq GV (the part of
+// A22 X it had not seen): one product [V | Vq | Zq] [M/2; GZ; GV] per 64-row
+// block on MFMA, wave v owning rows 16 v .. 16 v + 15.
 __global__ __launch_bounds__(256) void k_sbr_z(int m, const double *__restrict__ Y, long ldy,
                                                const double *__restrict__ M,
-                                               double *__restrict__ VW, long ldv) {
-  __shared__ double Ms[B][B + 1];
-  const int t = threadIdx.x;
-  for (int e = t; e < B * B; e += 256) Ms[e / B][e % B] = M[e];
+                                               const double *__restrict__ V,
+                                               double *__restrict__ Z, long ldv,
+                                               const double *__restrict__ Vq,
+                                               const double *__restrict__ Zq, long ldq) {
+  constexpr int K = 3 * B;
+  __shared__ double As[64][K + 1];  // [V | Vq | Zq] rows of the block
+  __shared__ double Cs[K][B + 1];   // [M/2; GZ; GV]
+  const int t = threadIdx.x, r = t & 63, cg = t >> 6;
+  const int row0 = 64 * blockIdx.x, row = row0 + r;
+  const bool pend = Vq != nullptr;
+  const int kk = pend ? K : B;
+  for (int e = t; e < kk * B; e += 256) {
+    const int i = e / B, k = e % B;
+    // M row-major at M[0..), GV at M[B*B..), GZ at M[2*B*B..)
+    Cs[i][k] = (i < B) ? 0.5 * M[i * B + k]
+                       : (i < 2 * B ? M[2 * B * B + (i - B) * B + k] : M[B * B + (i - 2 * B) * B + k]);
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = 8 * cg + j;
+    As[r][c] = (row < m) ? V[row + (long)c * ldv] : 0.0;
+    if (pend) {
+      As[r][B + c] = (row < m) ? Vq[row + (long)c * ldq] : 0.0;
+      As[r][2 * B + c] = (row < m) ? Zq[row + (long)c * ldq] : 0.0;
+    }
+  }
   __syncthreads();
-  const int row = blockIdx.x * 256 + t;
-  if (row >= m) return;
-  double v[B];
+  const int lane = t & 63, lr = lane & 15, lk = lane >> 4;
+  v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  for (int k4 = 0; k4 < kk / 4; ++k4) {
+    const double a = As[16 * cg + lr][4 * k4 + lk];
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Cs[4 * k4 + lk][lr], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Cs[4 * k4 + lk][16 + lr], acc1, 0, 0, 0);
+  }
 #pragma unroll
-  for (int i = 0; i < B; ++i) v[i] = VW[row + (long)i * ldv];
-#pragma unroll 4
-  for (int k = 0; k < B; ++k) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < B; ++i) s += v[i] * Ms[i][k];
-    VW[row + (long)(B + k) * ldv] = Y[row + (long)k * ldy] - 0.5 * s;
+  for (int v = 0; v < 4; ++v) {
+    const int rr = row0 + 16 * cg + lk + 4 * v;
+    if (rr < m) {
+      Z[rr + (long)lr * ldv] = Y[rr + (long)lr * ldy] - acc0[v];
+      Z[rr + (long)(16 + lr) * ldv] = Y[rr + (long)(16 + lr) * ldy] - acc1[v];
+    }
   }
 }
 
@@ -1176,7 +1245,7 @@ SbrSizes sbr_sizes(int n) {
   z.X = (size_t)n * B;
   z.Yp = (size_t)kSbrMaxSplit * B * n;
   const int nb = ceil_div(n, 64);
-  z.Mp = ((size_t)nb + ceil_div(nb, 16) + 1) * B * B;
+  z.Mp = ((size_t)nb + ceil_div(nb, 16) + 1) * 3 * B * B;
   z.mcnt = (size_t)ceil_div(nb, 16) + 2;
   z.AB = (size_t)n * LDB;
   z.prog = (size_t)n;
@@ -1197,10 +1266,31 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
   hipLaunchKernelGGL(k_sbr_zero, dim3(ceil_div((long)std::max(std::max(z.cnt, z.mcnt), z.cq), 256)),
                      dim3(256), 0, s, (int)z.cnt, w.cnt, (int)z.mcnt, w.mcnt, (int)z.cq, w.cq);
   MBD_LAUNCH_CHECK();
+  // Panels are applied to the trailing matrix in pairs: the first panel's
+  // update stays pending, the second applies it to its own columns, corrects
+  // its symmetric product for it, and one K = 4B SYR2K applies both.  Work
+  // buffer columns: [V_a | V_b | Z_a | Z_b | V_a | V_b] (rows of the first
+  // panel's trailing block; the second panel's rows start B lower).
+  const long ldv = w.ldv;
+  double *const Va = w.VW, *const Za = w.VW + 2L * B * ldv;
   int c = 0, pw = (n % B) ? n % B : B, panel = 0;
+  bool pend = false;
   while (c + pw < n) {
     const int r0 = c + pw, m = n - r0;
     const Tree tr = make_tree(m);
+    const bool second = pend;
+    double *const Wb = w.VW + (second ? B : 0);
+    double *const V1 = Wb + (long)(second ? 1 : 0) * B * ldv;
+    double *const V2 = Wb + (long)(second ? 5 : 4) * B * ldv;
+    double *const Zc = Wb + (long)(second ? 3 : 2) * B * ldv;
+    if (second) {
+      // the pending update on this panel's columns (rows c.., the first panel's rows 0..)
+      ProfScope ps(K_TRD_SYR2K, s, 0.0, 4.0 * B * (double)(n - c) * pw);
+      gemm_nt(s, false, n - c, pw, B, -1.0, Va, (int)ldv, 0, Za, (int)ldv, 0, 1.0,
+              A + c + (long)c * lda, (int)lda, 0, 1);
+      gemm_nt(s, false, n - c, pw, B, -1.0, Za, (int)ldv, 0, Va, (int)ldv, 0, 1.0,
+              A + c + (long)c * lda, (int)lda, 0, 1);
+    }
     int *fail = w.cq + 2 + panel;  // 0: CholeskyQR2 factored the panel, 1: TSQR does
     {
       ProfScope ps(K_SBR_QR, s, 8.0 * 2.0 * m * B, 4.0 * (double)m * B * B);
@@ -1212,13 +1302,13 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
                          w.Mp, w.cq + 1, w.cqR, fail);
       MBD_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_sbr_cq3, dim3(tr.L[0]), dim3(NT), 0, s, m, w.Yp, (long)n, w.cqR, fail,
-                         w.VW, w.ldv, w.X, (long)n, P, lda);
+                         V1, V2, ldv, w.X, (long)n, P, lda);
       MBD_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_sbr_tsqr, dim3(tr.L[0]), dim3(NT), 0, s, m, pw, P, lda, w.Vn, w.Tn,
                          w.Rn, w.cnt, tr, fail);
       MBD_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_sbr_qform, dim3(tr.L[0]), dim3(NT), 0, s, m, pw, w.Vn, w.Tn, w.Rn,
-                         tr, w.VW, w.ldv, w.X, (long)n, P, lda, fail);
+                         tr, V1, V2, ldv, w.X, (long)n, P, lda, fail);
       MBD_LAUNCH_CHECK();
     }
     const int nb = ceil_div(m, 64);
@@ -1232,20 +1322,32 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
                          lda, w.X, (long)n, w.Yp, (long)n, kchunk);
       MBD_LAUNCH_CHECK();
     }
+    // pending factors on this panel's rows (B below the first panel's row 0)
+    const double *Vq = second ? Va + B : nullptr, *Zq = second ? Za + B : nullptr;
     hipLaunchKernelGGL(k_sbr_zred, dim3(nb), dim3(256), 0, s, m, ks, w.Yp, (long)n, w.X,
-                       (long)n, w.Mp, w.mcnt, w.M);
+                       (long)n, Vq, Zq, ldv, w.Mp, w.mcnt, w.M);
     MBD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_sbr_z, dim3(ceil_div(m, 256)), dim3(256), 0, s, m, w.Yp, (long)n, w.M,
-                       w.VW, w.ldv);
+    hipLaunchKernelGGL(k_sbr_z, dim3(nb), dim3(256), 0, s, m, w.Yp, (long)n, w.M, V1, Zc, ldv,
+                       Vq, Zq, ldv);
     MBD_LAUNCH_CHECK();
-    {
-      ProfScope ps(K_TRD_SYR2K, s, 0.0, 2.0 * B * (double)m * (m + 1.0));
-      gemm_nt(s, true, m, m, 2 * B, -1.0, w.VW, (int)w.ldv, 0, w.VW + (long)B * w.ldv,
-              (int)w.ldv, 0, 1.0, A + r0 + (long)r0 * lda, (int)lda, 0, 1);
+    if (second) {
+      ProfScope ps(K_TRD_SYR2K, s, 0.0, 4.0 * B * (double)m * (m + 1.0));
+      gemm_nt(s, true, m, m, 4 * B, -1.0, w.VW + B, (int)ldv, 0, w.VW + B + 2L * B * ldv,
+              (int)ldv, 0, 1.0, A + r0 + (long)r0 * lda, (int)lda, 0, 1);
     }
+    pend = !second;
     c = r0;
     pw = B;
     ++panel;
+  }
+  if (pend) {
+    // the last panel's update on its trailing block: two rank-B products on the lower tiles
+    const int m = n - c;
+    ProfScope ps(K_TRD_SYR2K, s, 0.0, 2.0 * B * (double)m * (m + 1.0));
+    gemm_nt(s, true, m, m, B, -1.0, Va, (int)ldv, 0, Za, (int)ldv, 0, 1.0,
+            A + c + (long)c * lda, (int)lda, 0, 1);
+    gemm_nt(s, true, m, m, B, -1.0, Za, (int)ldv, 0, Va, (int)ldv, 0, 1.0,
+            A + c + (long)c * lda, (int)lda, 0, 1);
   }
   const SbrSizes zz = sbr_sizes(n);
   hipLaunchKernelGGL(k_sbr_band, dim3(std::min(ceil_div((long)std::max<size_t>(zz.AB, zz.mail), 256), 8192)),
