@@ -549,6 +549,38 @@ __device__ inline void block_gram(const double (*Ps)[NT + 1], double *Gp) {
   for (int v = 0; v < 4; ++v) Gp[(16 * ti + lk + 4 * v) * B + 16 * tj + lr] = acc[v];
 }
 
+// Rows of the block times a B x B matrix on MFMA: out[r][c] = sum_k Ls[k][r] Mt[k][c]
+// for the block's NT rows (Ls[col][row] in LDS), wave v owning rows 64 v .. 64 v + 63.
+// Results go to out (global, ld, rows row0.. below m) and, if Os is non-null,
+// to Os[col][row] in LDS.
+__device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, double *out, long ld,
+                               int row0, int m, double (*Os)[NT + 1]) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 15, lk = lane >> 4;
+  v4d acc[4][2];
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) acc[rt][0] = acc[rt][1] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s4 = 0; s4 < B / 4; ++s4) {
+    const double b0 = Mt[4 * s4 + lk][lr], b1 = Mt[4 * s4 + lk][16 + lr];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      const double a = Ls[4 * s4 + lk][64 * wv + 16 * rt + lr];
+      acc[rt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[rt][0], 0, 0, 0);
+      acc[rt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[rt][1], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rl = 64 * wv + 16 * rt + lk + 4 * v, c = 16 * ct + lr;
+        if (row0 + rl < m) out[row0 + rl + (long)c * ld] = acc[rt][ct][v];
+        if (Os) Os[c][rl] = acc[rt][ct][v];
+      }
+}
+
 // Last of the grid's blocks to arrive sums the partial Grams into G (LDS).
 __device__ inline bool gram_reduce_last(const double *Gp, int *cnt, Mat32 *G) {
   __shared__ int last;
@@ -561,6 +593,7 @@ __device__ inline bool gram_reduce_last(const double *Gp, int *cnt, Mat32 *G) {
   if (threadIdx.x == 0) *cnt = 0;
   for (int e = threadIdx.x; e < B * B; e += NT) {
     double s = 0.0;
+#pragma unroll 8
     for (int b = 0; b < (int)gridDim.x; ++b) s += Gp[(long)b * B * B + e];
     G[e / B][e % B] = s;
   }
@@ -597,22 +630,18 @@ __global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__re
   if (*fail) return;
   __shared__ double Ps[B][NT + 1];
   __shared__ Mat32 R[B], Ri[B], G[B];
+  __shared__ double Qs[B][NT + 1];
   const int t = threadIdx.x, row = NT * blockIdx.x + t;
   for (int e = t; e < B * B; e += NT) R[e / B][e % B] = Rbuf[e];
+#pragma unroll
+  for (int k = 0; k < B; ++k) Ps[k][t] = (row < m && k < w) ? P[row + (long)k * lda] : 0.0;
   __syncthreads();
   lds_inv_upper(R, Ri);
-  double pr[B];
-#pragma unroll
-  for (int k = 0; k < B; ++k) pr[k] = (row < m && k < w) ? P[row + (long)k * lda] : 0.0;
-  for (int k = 0; k < B; ++k) {
-    double q = 0.0;
-#pragma unroll
-    for (int i = 0; i < B; ++i) q += pr[i] * Ri[i][k];
-    Ps[k][t] = q;
-    if (row < m) Q1[row + (long)k * ldq] = q;
-  }
+  // Q1 = P R1^{-1} for the block's rows, to global and to LDS for the Gram
+  rows_mm(Ps, Ri, Q1, ldq, NT * blockIdx.x, m, Qs);
   __syncthreads();
-  block_gram(Ps, Gp + (long)blockIdx.x * B * B);
+  const double (*Gs)[NT + 1] = Qs;
+  block_gram(Gs, Gp + (long)blockIdx.x * B * B);
   if (!gram_reduce_last(Gp, cnt, G)) return;
   double dev = 0.0;
   for (int e = 0; e < B * B; ++e) dev = fmax(dev, fabs(G[e / B][e % B] - (e / B == e % B ? 1.0 : 0.0)));
@@ -632,6 +661,7 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
                                                 double *__restrict__ Ab, long lda) {
   if (*fail) return;
   __shared__ Mat32 R2[B], R2i[B], Wm[B], U[B], Y1[B], Ui[B], Yi[B], Tf[B], My[B], Mx[B];
+  __shared__ double Qb[B][NT + 1];  // the block's rows of Q1
   __shared__ double sgn[B];
   const int t = threadIdx.x;
   for (int e = t; e < B * B; e += NT) {
@@ -688,9 +718,18 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
   lds_mm(My, R2i, Ui, 1.0);
   lds_mm(Mx, My, Tf, 1.0);
   const int g = NT * blockIdx.x + t;
-  if (g < m) {
+  // y = q1 My (into V1 and V2), x = q1 Mx, for every row on MFMA; the top
+  // block's rows are then replaced by their exact form below
+#pragma unroll
+  for (int k = 0; k < B; ++k) Qb[k][t] = (g < m) ? Q1[g + (long)k * ldq] : 0.0;
+  __syncthreads();
+  rows_mm(Qb, My, V1, ldv, NT * blockIdx.x, m, nullptr);
+  rows_mm(Qb, My, V2, ldv, NT * blockIdx.x, m, nullptr);
+  rows_mm(Qb, Mx, X, ldx, NT * blockIdx.x, m, nullptr);
+  __syncthreads();  // the product's stores to the top rows land before their replacement
+  if (g < m && g < B) {
     double qr[B];
-    if (g < B) {
+    {
 #pragma unroll
       for (int k = 0; k < B; ++k) qr[k] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
       for (int k = 0; k < B; ++k) {
@@ -699,20 +738,6 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
         for (int i = 0; i < B; ++i) x += qr[i] * Tf[i][k];
         V1[g + (long)k * ldv] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
         V2[g + (long)k * ldv] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
-        X[g + (long)k * ldx] = x;
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < B; ++k) qr[k] = Q1[g + (long)k * ldq];
-      for (int k = 0; k < B; ++k) {
-        double y = 0.0, x = 0.0;
-#pragma unroll
-        for (int i = 0; i < B; ++i) {
-          y += qr[i] * My[i][k];
-          x += qr[i] * Mx[i][k];
-        }
-        V1[g + (long)k * ldv] = y;
-        V2[g + (long)k * ldv] = y;
         X[g + (long)k * ldx] = x;
       }
     }
@@ -829,17 +854,29 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
   const int r = t & 63, cg = t >> 6;
   const int row = 64 * I + r;
   const bool pend = Vq != nullptr;
+  {
+    // every load first, then the stores: a store into Yp between the loads
+    // would order each column's loads behind the previous column's store
+    double ys[8], xv[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int c = 8 * cg + j;
-    double s = 0.0, xv = 0.0;
-    if (row < m) {
-      for (int q = 0; q < nsplit; ++q) s += Yp[(long)q * B * ldy + row + (long)c * ldy];
-      Yp[row + (long)c * ldy] = s;
-      xv = X[row + (long)c * ldx];
+    for (int j = 0; j < 8; ++j) {
+      const int c = 8 * cg + j;
+      double s = 0.0;
+      if (row < m) {
+#pragma unroll
+        for (int q = 0; q < kSbrMaxSplit; ++q)
+          if (q < nsplit) s += Yp[(long)q * B * ldy + row + (long)c * ldy];
+      }
+      ys[j] = s;
+      xv[j] = (row < m) ? X[row + (long)c * ldx] : 0.0;
     }
-    Ys[c][r] = s;
-    Xs[c][r] = xv;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int c = 8 * cg + j;
+      if (row < m) Yp[row + (long)c * ldy] = ys[j];
+      Ys[c][r] = ys[j];
+      Xs[c][r] = xv[j];
+    }
   }
   __syncthreads();
   // 32 x 32 products over the block's 64 rows on MFMA: wave v owns tile (v >> 1, v & 1)
@@ -878,10 +915,22 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
   __threadfence();
   if (t == 0) mcnt[1 + grp] = 0;
   double *Mg = Mp + (long)nb * NQ;
-  for (int e = t; e < nq; e += 256) {
-    double s = 0.0;
-    for (int q = 0; q < gsz; ++q) s += Mp[(long)(grp * G + q) * NQ + e];
-    Mg[(long)grp * NQ + e] = s;
+  {
+    double acc[NQ / 256];
+#pragma unroll
+    for (int i = 0; i < NQ / 256; ++i) {
+      const int e = t + 256 * i;
+      double s = 0.0;
+      if (e < nq) {
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          if (q < gsz) s += Mp[(long)(grp * G + q) * NQ + e];
+      }
+      acc[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < NQ / 256; ++i)
+      if (t + 256 * i < nq) Mg[(long)grp * NQ + t + 256 * i] = acc[i];
   }
   __threadfence();
   __syncthreads();
@@ -892,15 +941,26 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
   if (t == 0) mcnt[0] = 0;
   // GV and GZ also go to LDS (Ys, Xs are free) for the correction of M
   double *gv = &Ys[0][0], *gz = &Xs[0][0];
-  for (int e = t; e < nq; e += 256) {
+  double acc[NQ / 256];
+#pragma unroll
+  for (int i = 0; i < NQ / 256; ++i) {
+    const int e = t + 256 * i;
     double s = 0.0;
-    for (int g = 0; g < ng; ++g) s += Mg[(long)g * NQ + e];
+    if (e < nq) {
+#pragma unroll 4
+      for (int g = 0; g < ng; ++g) s += Mg[(long)g * NQ + e];
+    }
+    acc[i] = s;
+  }
+#pragma unroll
+  for (int i = 0; i < NQ / 256; ++i) {
+    const int e = t + 256 * i;
+    if (e >= nq) continue;
     if (e >= 2 * B * B)
-      gz[e - 2 * B * B] = s;
+      gz[e - 2 * B * B] = acc[i];
     else if (e >= B * B)
-      gv[e - B * B] = s;
-    if (!pend || e >= B * B) M[e] = s;
-    if (pend && e < B * B) Mp[e] = s;  // uncorrected X^T Y, in a partial slot already consumed
+      gv[e - B * B] = acc[i];
+    if (!pend || e >= B * B) M[e] = acc[i];
   }
   if (pend) {
     __syncthreads();
@@ -911,7 +971,7 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
       double s = 0.0;
 #pragma unroll 8
       for (int l = 0; l < B; ++l) s += gv[l * B + a] * gz[l * B + b] + gz[l * B + a] * gv[l * B + b];
-      M[e] = Mp[e] - s;
+      M[e] = acc[i] - s;
     }
   }
 }
