@@ -66,20 +66,17 @@ def roofline_entry(prof, cls, bound):
             / max(k["launches"], 1)}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trd_symv_traffic_n4096.md")
-
-
 def measured_traffic_ratio(kernel):
     """HBM bytes per launch of `kernel` over its algorithmic bytes, from the
-    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled
-    by what the counter reads for a 1 GiB copy), or None when no summary
-    covers the kernel."""
-    if not os.path.exists(TRAFFIC_FILE):
-        return None
-    for line in open(TRAFFIC_FILE):
-        t = line.strip().strip("`").split()
-        if len(t) == 3 and t[0] == f"traffic_ratio[{kernel}]" and t[1] == "=":
-            return float(t[2])
+    committed rocprofv3 --pmc summaries (profiles/*traffic*.md: FETCH_SIZE +
+    WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy), or
+    None when no summary covers the kernel."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
+        for line in open(path):
+            t = line.strip().strip("`").split()
+            if len(t) == 3 and t[0] == f"traffic_ratio[{kernel}]" and t[1] == "=":
+                return float(t[2])
     return None
 
 
@@ -99,6 +96,55 @@ def traffic_probe(args):
     A = np.tril(A) + np.tril(A, -1).T
     w = engine.device_sym_eigvals(A)
     log(f"traffic-probe: eigen stage n={n} done, trace {w.sum():.6e} vs {np.trace(A):.6e}")
+
+
+def batched_line(steps, warmup):
+    """Batched form: 512 chain molecules (8-256 atoms) per launch, mixed
+    precision, default launch shape, hipGraph replays; the FP64 batch and a
+    sample of the single-system path alongside."""
+    from mbd_amd import engine, generators
+    mols, st = generators.chain_molecules()
+    eng = engine.MbdEngine(0)
+    eng.batch_set(mols, st)
+    out = {"workload": "c4_chains_512: 512 non-periodic chains, 8-256 atoms, all 21 points",
+           "molecules": len(mols), "atoms": int(sum(m.n_atoms for m in mols))}
+    res = {}
+    for mode in ("mixed", "fp64"):
+        for _ in range(warmup):
+            eng.batch_run(mode, full=False)
+        t0 = time.perf_counter()
+        dev = 0.0
+        for _ in range(steps):
+            r = eng.batch_run(mode, full=False)
+            dev += r["diag"]["t_ms"]
+        dt = (time.perf_counter() - t0) / steps
+        res[mode] = eng.batch_run(mode)
+        out["launch_shape"] = {"index": res[mode]["diag"]["config"],
+                               **engine.batch_config(res[mode]["diag"]["config"])}
+        out[mode] = {"ms_per_batch": 1e3 * dt, "device_ms_per_batch": dev / steps,
+                     "molecules_per_s": len(mols) / dt,
+                     "workgroup_ms_by_phase": {k: round(v, 2) for k, v in
+                                               res[mode]["diag"]["phase_ms"].items()},
+                     "fallbacks": res[mode]["diag"]["fallbacks"],
+                     "max_iters": res[mode]["diag"]["max_iters"]}
+        log(f"bench: batched {mode} {1e3 * dt:.2f} ms per 512 molecules")
+    out["error_vs_fp64"] = {
+        "e_mbd_ha_max": float(np.max(np.abs(res["mixed"]["energy"] - res["fp64"]["energy"]))),
+        "c6_rel_max": float(np.max(np.abs(res["mixed"]["c6_eff"] / res["fp64"]["c6_eff"] - 1)))}
+    eng.close()
+    # single-system path on every 8th molecule, scaled to the batch
+    single = engine.MbdEngine(0)
+    t_single = 0.0
+    for g in mols[::8]:
+        single.set_geometry(g, st)
+        single.run_energy_only("mixed")
+        t0 = time.perf_counter()
+        single.run_energy_only("mixed")
+        t_single += time.perf_counter() - t0
+    single.close()
+    out["single_system_path_s_per_batch_est"] = 8 * t_single
+    out["speedup_vs_single_system_path"] = 8 * t_single / (out["mixed"]["ms_per_batch"] * 1e-3)
+    return out
 
 
 def cpu_baseline(geo, st, sample_eig_n=3072):
