@@ -202,6 +202,11 @@ struct mbd_handle {
   DevBuf<unsigned long long> sbrMail;
   DevBuf<int> sbrCq;
   DevBuf<double> sbrCqR;
+  // look-ahead: second factor buffer, the stream the pair updates run on, and
+  // the events that fork to it and join from it
+  DevBuf<double> sbrVW2;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t sbr_fork = nullptr, sbr_join = nullptr;
   int eig_used = 1;
   int eig_flag = 0;
   // instrumentation
@@ -506,6 +511,16 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
     sw = SbrWork{h->sbrV.p, h->sbrT.p, h->sbrR.p, h->sbrCnt.p, h->sbrX.p, h->sbrY.p,
                  h->sbrMp.p, h->sbrM.p, h->sbrMcnt.p, h->sbrAB.p, h->sbrProg.p, h->sbrFlag.p,
                  h->sbrMail.p, h->sbrCq.p, h->sbrCqR.p, h->VW.p, nm, h->dd.p, h->ee.p};
+    if (sbr_lookahead()) {
+      h->sbrVW2.ensure((size_t)nm * 6 * kSbrB);
+      if (!h->stream2) MBD_HIP(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+      if (!h->sbr_fork) MBD_HIP(hipEventCreateWithFlags(&h->sbr_fork, hipEventDisableTiming));
+      if (!h->sbr_join) MBD_HIP(hipEventCreateWithFlags(&h->sbr_join, hipEventDisableTiming));
+      sw.VW2 = h->sbrVW2.p;
+      sw.s2 = h->stream2;
+      sw.fork = h->sbr_fork;
+      sw.join = h->sbr_join;
+    }
   }
   auto reduce = [&]() {
     if (method == 2)
@@ -522,7 +537,7 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
                                            h->pa.p, h->pb.p, h->py.p, h->qred.p, h->ps.p,
                                            sw.Vn, sw.Tn, sw.Rn, sw.cnt, sw.X, sw.Yp, sw.Mp, sw.M,
                                            sw.mcnt, sw.AB, sw.prog, sw.flag, sw.mail, sw.cq,
-                                           sw.cqR};
+                                           sw.cqR, sw.VW2, (const void *)sw.s2};
     if (!h->trd_exec || h->trd_key != key) {
       if (h->trd_exec) (void)hipGraphExecDestroy(h->trd_exec);
       h->trd_exec = nullptr;
@@ -651,8 +666,18 @@ int mbd_destroy(mbd_handle *h) {
   h->info.release();
   h->cell.release();
   h->nneg.release();
+  DevBuf<double> *sbufs[] = {&h->sbrV, &h->sbrT, &h->sbrR, &h->sbrX, &h->sbrY, &h->sbrMp,
+                             &h->sbrM, &h->sbrAB, &h->sbrCqR, &h->sbrVW2};
+  for (auto *b : sbufs) b->release();
+  DevBuf<int> *sibufs[] = {&h->sbrCnt, &h->sbrMcnt, &h->sbrProg, &h->sbrFlag, &h->sbrCq};
+  for (auto *b : sibufs) b->release();
+  h->sbrMail.release();
   batch_release(h);
   if (h->trd_exec) (void)hipGraphExecDestroy(h->trd_exec);
+  if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+  if (h->sbr_fork) (void)hipEventDestroy(h->sbr_fork);
+  if (h->sbr_join) (void)hipEventDestroy(h->sbr_join);
+  if (h->stream2) (void)hipStreamDestroy(h->stream2);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return 0;
@@ -1239,11 +1264,20 @@ int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w) {
   if (!A) return -3;
   if (!w) return -4;
   std::vector<double *> bufs;
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
   auto alloc = [&](size_t cnt) {
     double *p = nullptr;
     MBD_HIP(hipMalloc(&p, sizeof(double) * std::max<size_t>(cnt, 1)));
     bufs.push_back(p);
     return p;
+  };
+  auto release = [&]() {
+    (void)hipDeviceSynchronize();
+    for (auto *p : bufs) (void)hipFree(p);
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (s2) (void)hipStreamDestroy(s2);
   };
   try {
     MBD_HIP(hipSetDevice(device));
@@ -1278,6 +1312,15 @@ int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w) {
                  alloc(z.Mp), alloc((size_t)3 * kSbrB * kSbrB), ialloc(z.mcnt), alloc(z.AB),
                  ialloc(z.prog), ialloc(4), reinterpret_cast<unsigned long long *>(alloc(z.mail)),
                  ialloc(z.cq), alloc(z.cqR), wk.VW, wk.ldv, wk.d, wk.e};
+      if (sbr_lookahead()) {
+        sw.VW2 = alloc((size_t)n * 6 * kSbrB);
+        MBD_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        MBD_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        MBD_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        sw.s2 = s2;
+        sw.fork = ev[0];
+        sw.join = ev[1];
+      }
       sbr_tridiagonalize(nullptr, n, dA, n, sw);
       MBD_HIP(hipMemcpy(&abandoned, sw.flag, sizeof(int), hipMemcpyDeviceToHost));
     } else {
@@ -1288,14 +1331,14 @@ int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w) {
     MBD_HIP(hipDeviceSynchronize());
     MBD_HIP(hipMemcpy(w, lam, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (abandoned) {
-      for (auto *p : bufs) (void)hipFree(p);
+      release();
       return MBD_ERR_EIG;
     }
   } catch (const HipError &e) {
-    for (auto *p : bufs) (void)hipFree(p);
+    release();
     return fail(nullptr, e);
   }
-  for (auto *p : bufs) (void)hipFree(p);
+  release();
   return 0;
 }
 

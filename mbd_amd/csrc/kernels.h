@@ -88,6 +88,12 @@ struct SbrWork {
   double *VW;            // two panels' factors [V_a | V_b | Z_a | Z_b | V_a | V_b], n x 6B (ld ldv)
   long ldv;
   double *d, *e;         // tridiagonal output
+  // look-ahead (optional): the SYR2K of a panel pair runs on s2 while the next
+  // panel is factored on the main stream; consecutive pairs alternate between
+  // VW and VW2 (same layout).  s2 == nullptr applies every update in order.
+  double *VW2;
+  hipStream_t s2;
+  hipEvent_t fork, join;
 };
 // element counts of the work buffers for order n
 struct SbrSizes {
@@ -96,6 +102,8 @@ struct SbrSizes {
 SbrSizes sbr_sizes(int n);
 int sbr_mail_tasks(int n);
 void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w);
+// look-ahead of the two-stage reduction on (default) unless MBD_SBR_LOOKAHEAD=0
+bool sbr_lookahead();
 // eigen-reduction method for order n: 1 = one-stage (tridiagonalize), 2 = two-stage;
 // MBD_EIG_METHOD=1|2 forces one
 int eig_method(int n);

@@ -1315,6 +1315,11 @@ SbrSizes sbr_sizes(int n) {
   return z;
 }
 
+bool sbr_lookahead() {
+  const char *e = getenv("MBD_SBR_LOOKAHEAD");
+  return !(e && e[0] == '0');
+}
+
 int eig_method(int n) {
   const char *e = getenv("MBD_EIG_METHOD");
   if (e && (e[0] == '1' || e[0] == '2')) return e[0] - '0';
@@ -1331,15 +1336,22 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
   // its symmetric product for it, and one K = 4B SYR2K applies both.  Work
   // buffer columns: [V_a | V_b | Z_a | Z_b | V_a | V_b] (rows of the first
   // panel's trailing block; the second panel's rows start B lower).
+  // Look-ahead (w.s2 set): a pair's SYR2K runs on s2 over the trailing block
+  // less its first B columns, which the main stream updates itself and then
+  // factors as the next panel; the two streams join before that panel's
+  // symmetric product, and consecutive pairs alternate factor buffers.
   const long ldv = w.ldv;
-  double *const Va = w.VW, *const Za = w.VW + 2L * B * ldv;
+  const bool la = w.s2 != nullptr && w.VW2 != nullptr;
+  double *VWc = w.VW;  // factor buffer of the current pair
+  bool joined = true;  // false while a pair's SYR2K may still run on s2
   int c = 0, pw = (n % B) ? n % B : B, panel = 0;
   bool pend = false;
   while (c + pw < n) {
     const int r0 = c + pw, m = n - r0;
     const Tree tr = make_tree(m);
     const bool second = pend;
-    double *const Wb = w.VW + (second ? B : 0);
+    double *const Va = VWc, *const Za = VWc + 2L * B * ldv;
+    double *const Wb = VWc + (second ? B : 0);
     double *const V1 = Wb + (long)(second ? 1 : 0) * B * ldv;
     double *const V2 = Wb + (long)(second ? 5 : 4) * B * ldv;
     double *const Zc = Wb + (long)(second ? 3 : 2) * B * ldv;
@@ -1371,6 +1383,12 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
                          tr, V1, V2, ldv, w.X, (long)n, P, lda, fail);
       MBD_LAUNCH_CHECK();
     }
+    if (!joined) {
+      // the previous pair's SYR2K has to finish before A22 is read
+      MBD_HIP(hipEventRecord(w.join, w.s2));
+      MBD_HIP(hipStreamWaitEvent(s, w.join, 0));
+      joined = true;
+    }
     const int nb = ceil_div(m, 64);
     int ks = std::max(1, std::min(kSbrMaxSplit, 512 / nb));
     const int kchunk = ceil_div(ceil_div(m, ks), SY_K) * SY_K;
@@ -1390,9 +1408,28 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
     hipLaunchKernelGGL(k_sbr_z, dim3(nb), dim3(256), 0, s, m, w.Yp, (long)n, w.M, V1, Zc, ldv,
                        Vq, Zq, ldv);
     MBD_LAUNCH_CHECK();
-    if (second) {
+    if (second && la && r0 + B < n) {
+      // the next panel (columns r0 .. r0 + B) exists: its columns here, the
+      // rest of the trailing triangle on s2
+      MBD_HIP(hipEventRecord(w.fork, s));
+      MBD_HIP(hipStreamWaitEvent(w.s2, w.fork, 0));
+      {
+        const int m2 = m - B;
+        ProfScope ps(K_TRD_SYR2K, w.s2, 0.0, 4.0 * B * (double)m2 * (m2 + 1.0));
+        gemm_nt(w.s2, true, m2, m2, 4 * B, -1.0, VWc + 2 * B, (int)ldv, 0,
+                VWc + 2 * B + 2L * B * ldv, (int)ldv, 0, 1.0, A + (r0 + B) + (long)(r0 + B) * lda,
+                (int)lda, 0, 1);
+      }
+      joined = false;
+      {
+        ProfScope ps(K_TRD_SYR2K, s, 0.0, 8.0 * B * (double)m * B);
+        gemm_nt(s, false, m, B, 4 * B, -1.0, VWc + B, (int)ldv, 0, VWc + B + 2L * B * ldv,
+                (int)ldv, 0, 1.0, A + r0 + (long)r0 * lda, (int)lda, 0, 1);
+      }
+      VWc = (VWc == w.VW) ? w.VW2 : w.VW;
+    } else if (second) {
       ProfScope ps(K_TRD_SYR2K, s, 0.0, 4.0 * B * (double)m * (m + 1.0));
-      gemm_nt(s, true, m, m, 4 * B, -1.0, w.VW + B, (int)ldv, 0, w.VW + B + 2L * B * ldv,
+      gemm_nt(s, true, m, m, 4 * B, -1.0, VWc + B, (int)ldv, 0, VWc + B + 2L * B * ldv,
               (int)ldv, 0, 1.0, A + r0 + (long)r0 * lda, (int)lda, 0, 1);
     }
     pend = !second;
@@ -1400,9 +1437,14 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
     pw = B;
     ++panel;
   }
+  if (!joined) {
+    MBD_HIP(hipEventRecord(w.join, w.s2));
+    MBD_HIP(hipStreamWaitEvent(s, w.join, 0));
+  }
   if (pend) {
     // the last panel's update on its trailing block: two rank-B products on the lower tiles
     const int m = n - c;
+    double *const Va = VWc, *const Za = VWc + 2L * B * ldv;
     ProfScope ps(K_TRD_SYR2K, s, 0.0, 2.0 * B * (double)m * (m + 1.0));
     gemm_nt(s, true, m, m, B, -1.0, Va, (int)ldv, 0, Za, (int)ldv, 0, 1.0,
             A + c + (long)c * lda, (int)lda, 0, 1);

@@ -9,7 +9,11 @@ csrc/sbr.hip) through the C ABI, forced with MBD_EIG_METHOD=2:
   (diagonal, block diagonal, identity plus rank one), where reflectors with
   tau = 0 and the sign choice of the Householder reconstruction matter;
 * E_MBD of a 512-atom periodic cell equal to the one-stage reduction's within
-  1e-10 Ha, and the eager and graph-replayed launch forms bit-identical.
+  1e-10 Ha, and the eager and graph-replayed launch forms bit-identical;
+* the look-ahead schedule (a panel pair's SYR2K on a second stream while the
+  next panel is updated and factored) bit-identical to the in-order schedule:
+  every element of the trailing matrix gets the same products in the same
+  order, only on another stream.
 """
 import numpy as np
 import pytest
@@ -59,6 +63,20 @@ def test_two_stage_identity_plus_rank_one(monkeypatch):
     _check(np.eye(350) + np.outer(u, u), monkeypatch)
 
 
+@pytest.mark.parametrize("n", [130, 517, 1000, 2113])
+def test_two_stage_lookahead_bit_identical(n, monkeypatch):
+    rng = np.random.default_rng(4000 + n)
+    M = rng.standard_normal((n, n))
+    A = (M + M.T) / 2
+    monkeypatch.setenv("MBD_EIG_METHOD", "2")
+    monkeypatch.setenv("MBD_SBR_LOOKAHEAD", "0")
+    w0 = engine.device_sym_eigvals(A)
+    monkeypatch.setenv("MBD_SBR_LOOKAHEAD", "1")
+    w1 = engine.device_sym_eigvals(A)
+    print(f"n={n} max|w_la - w_inorder|={np.abs(w1 - w0).max():.2e}", flush=True)
+    assert np.array_equal(w0, w1)
+
+
 def test_two_stage_energy_matches_one_stage(monkeypatch):
     geo, st = generators.random_periodic(512, 19.0, seed=31, rc_scs=20.0, rc_sc=12.0,
                                          rc_cfdm=30.0)
@@ -91,3 +109,6 @@ def test_two_stage_forms_agree(monkeypatch):
             other.run("mixed")
             assert r.e_mbd == eager.e_mbd
             assert r.diag["n_negative"] == eager.diag["n_negative"]
+        monkeypatch.setenv("MBD_SBR_LOOKAHEAD", "0")
+        inorder = g.run("fp64")
+        assert inorder.e_mbd == eager.e_mbd
