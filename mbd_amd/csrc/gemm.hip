@@ -14,6 +14,11 @@
 //   B operand: lane l -> B^T[k l/16][col l%16]
 //   D (FP32):  lane l -> D[row 4*(l/16)+v][col l%16], v = 0..3
 //   D (FP64):  lane l -> D[row (l/16)+4*v][col l%16], v = 0..3
+// FP32 computes each 16 x 16 block transposed (the B tile as the MFMA's A
+// operand), so the 16 lanes of a group hold 16 consecutive rows of one column
+// of C and every epilogue access covers whole 64-byte column segments instead
+// of 16 scattered 16-byte pieces (measured faster for the FP32 Cholesky
+// updates; the FP64 updates measured slower that way and keep the direct form).
 #include "common.h"
 
 namespace mbd {
@@ -25,6 +30,7 @@ struct Mfma;
 template <>
 struct Mfma<double> {
   using acc_t = v4d;
+  static constexpr bool kSwap = false;
   // row of accumulator element v held by lane group g = lane / 16
   __device__ static int row(int g, int v) { return g + 4 * v; }
   __device__ static acc_t mma(double a, double b, acc_t c) {
@@ -34,6 +40,7 @@ struct Mfma<double> {
 template <>
 struct Mfma<float> {
   using acc_t = v4f;
+  static constexpr bool kSwap = true;
   __device__ static int row(int g, int v) { return 4 * g + v; }
   __device__ static acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -119,7 +126,9 @@ __global__ __launch_bounds__(NT) void k_gemm_nt(
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = Mfma<T>::kSwap ? Mfma<T>::mma(bf[j], af[i], acc[i][j])
+                                     : Mfma<T>::mma(af[i], bf[j], acc[i][j]);
     }
     if (kt + 1 < nk) sstore(cur ^ 1);
     __syncthreads();
@@ -131,8 +140,8 @@ __global__ __launch_bounds__(NT) void k_gemm_nt(
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = row0 + wm * 64 + i * 16 + Mfma<T>::row(lk, v);
-        const int c = col0 + wn * 64 + j * 16 + lr;
+        const int r = row0 + wm * 64 + i * 16 + (Mfma<T>::kSwap ? lr : Mfma<T>::row(lk, v));
+        const int c = col0 + wn * 64 + j * 16 + (Mfma<T>::kSwap ? Mfma<T>::row(lk, v) : lr);
         if (r < m && c < n && (!LOWER || r >= c)) {
           T *pc = C + r + (long)c * ldc;
           T val = alpha * acc[i][j][v];
