@@ -69,7 +69,7 @@ constexpr int kSbrLeaf = 256;      // rows of a TSQR tree node = threads of its 
 constexpr int kSbrFan = kSbrLeaf / kSbrB;  // children per TSQR tree node
 constexpr int kSbrLdb = 2 * kSbrB;  // band storage: diagonals 0 .. 2B-1 of each column
 constexpr int kSbrMaxLev = 8;
-constexpr int kSbrMaxSplit = 8;    // column splits of the symmetric product
+constexpr int kSbrMaxSplit = 16;   // column splits of the symmetric product
 constexpr int kSbrChaseWG = 256;   // workgroups of the bulge-chasing kernel
 struct SbrWork {
   double *Vn, *Tn, *Rn;  // TSQR nodes: reflectors (kSbrLeaf x B, row-major), T, R

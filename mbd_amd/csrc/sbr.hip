@@ -1390,7 +1390,10 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
       joined = true;
     }
     const int nb = ceil_div(m, 64);
-    int ks = std::max(1, std::min(kSbrMaxSplit, 512 / nb));
+    // column splits: about 2048 workgroups (8 per CU) so that the loads of
+    // other workgroups cover each one's memory latency (512 measured 54 ms per
+    // C3 solve, 2048 40 ms, 4096 no better)
+    int ks = std::max(1, std::min(kSbrMaxSplit, 2048 / nb));
     const int kchunk = ceil_div(ceil_div(m, ks), SY_K) * SY_K;
     ks = ceil_div(m, kchunk);
     {
