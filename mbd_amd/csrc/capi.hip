@@ -564,7 +564,7 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
     MBD_HIP(hipMemcpyAsync(&h->eig_flag, sw.flag, sizeof(int), hipMemcpyDeviceToHost, s));
   h->lam.ensure(nm);
   h->e2.ensure(nm);
-  h->bounds.ensure(4);
+  h->bounds.ensure(kBisectBounds);
   tridiag_eigvals(s, (int)nm, h->dd.p, h->ee.p, h->e2.p, h->bounds.p, h->lam.p);
   h->E.ensure(4);
   h->nneg.ensure(1);
@@ -1326,7 +1326,7 @@ int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w) {
     } else {
       tridiagonalize(nullptr, n, dA, n, wk);
     }
-    double *e2 = alloc(n), *bounds = alloc(4), *lam = alloc(n);
+    double *e2 = alloc(n), *bounds = alloc(kBisectBounds), *lam = alloc(n);
     tridiag_eigvals(nullptr, n, wk.d, wk.e, e2, bounds, lam);
     MBD_HIP(hipDeviceSynchronize());
     MBD_HIP(hipMemcpy(w, lam, sizeof(double) * n, hipMemcpyDeviceToHost));

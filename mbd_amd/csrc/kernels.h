@@ -109,6 +109,10 @@ bool sbr_lookahead();
 int eig_method(int n);
 // one-time kernel attributes of tridiagonalize (call before capturing it)
 void tridiag_prepare();
+// bounds: kBisectBounds doubles (Gershgorin interval, pivot guard, and the
+// Sturm counts of the starting grid)
+constexpr int kBisectGrid = 65536;  // equispaced points whose counts start every eigenvalue
+constexpr size_t kBisectBounds = 4 + kBisectGrid / 2 + 2;
 void tridiag_eigvals(hipStream_t s, int n, const double *d, const double *e,
                      double *e2, double *bounds, double *w);
 void energy(hipStream_t s, int n, const double *lam, int N,
