@@ -439,6 +439,28 @@ __global__ __launch_bounds__(NT) void k_sbr_qform(int m, int w, const double *__
 namespace {
 typedef double Mat32[B + 1];
 
+// Partial results handed to the last workgroup of a launch to arrive.  The
+// workgroups may sit on different XCDs, whose L2 caches are not coherent with
+// each other; instead of a device-scope fence (a write-back and an invalidate
+// of the whole L2 in every workgroup) the partials are written and read with
+// agent-scope atomics (coherent accesses), and each workgroup counts its
+// arrival once all its stores have completed.
+__device__ inline void st_coh(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double ld_coh(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// true in the last of `total` workgroups to call it on cnt
+__device__ inline bool arrive_last(int *cnt, int total, int *last) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) *last = (atomicAdd(cnt, 1) == total - 1);
+  __syncthreads();
+  return *last != 0;
+}
+
 // C = alpha A Bm, all 32 x 32 in LDS with explicit zeros (C distinct from A, Bm)
 __device__ inline void lds_mm(Mat32 *C, const Mat32 *A, const Mat32 *Bm, double alpha) {
   for (int e = threadIdx.x; e < B * B; e += NT) {
@@ -546,7 +568,7 @@ __device__ inline void block_gram(const double (*Ps)[NT + 1], double *Gp) {
     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
   }
 #pragma unroll
-  for (int v = 0; v < 4; ++v) Gp[(16 * ti + lk + 4 * v) * B + 16 * tj + lr] = acc[v];
+  for (int v = 0; v < 4; ++v) st_coh(&Gp[(16 * ti + lk + 4 * v) * B + 16 * tj + lr], acc[v]);
 }
 
 // Rows of the block times a B x B matrix on MFMA: out[r][c] = sum_k Ls[k][r] Mt[k][c]
@@ -584,17 +606,12 @@ __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, doub
 // Last of the grid's blocks to arrive sums the partial Grams into G (LDS).
 __device__ inline bool gram_reduce_last(const double *Gp, int *cnt, Mat32 *G) {
   __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = (atomicAdd(cnt, 1) == (int)gridDim.x - 1);
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+  if (!arrive_last(cnt, (int)gridDim.x, &last)) return false;
   if (threadIdx.x == 0) *cnt = 0;
   for (int e = threadIdx.x; e < B * B; e += NT) {
     double s = 0.0;
 #pragma unroll 8
-    for (int b = 0; b < (int)gridDim.x; ++b) s += Gp[(long)b * B * B + e];
+    for (int b = 0; b < (int)gridDim.x; ++b) s += ld_coh(&Gp[(long)b * B * B + e]);
     G[e / B][e % B] = s;
   }
   __syncthreads();
@@ -888,7 +905,7 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(L[16 * ti + lr][4 * k4 + lk], R[16 * tj + lr][4 * k4 + lk],
                                                 acc, 0, 0, 0);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) out[(16 * ti + lk + 4 * v) * B + 16 * tj + lr] = acc[v];
+    for (int v = 0; v < 4; ++v) st_coh(&out[(16 * ti + lk + 4 * v) * B + 16 * tj + lr], acc[v]);
   };
   tile(Xs, Ys, Mp + (long)I * NQ);
   if (pend) {
@@ -906,13 +923,8 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
     }
   }
   const int nq = pend ? NQ : B * B;
-  __threadfence();
-  __syncthreads();
   const int grp = I / G, ng = (nb + G - 1) / G, gsz = min(G, nb - grp * G);
-  if (t == 0) last = (atomicAdd(&mcnt[1 + grp], 1) == gsz - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+  if (!arrive_last(&mcnt[1 + grp], gsz, &last)) return;
   if (t == 0) mcnt[1 + grp] = 0;
   double *Mg = Mp + (long)nb * NQ;
   {
@@ -924,20 +936,15 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
       if (e < nq) {
 #pragma unroll
         for (int q = 0; q < G; ++q)
-          if (q < gsz) s += Mp[(long)(grp * G + q) * NQ + e];
+          if (q < gsz) s += ld_coh(&Mp[(long)(grp * G + q) * NQ + e]);
       }
       acc[i] = s;
     }
 #pragma unroll
     for (int i = 0; i < NQ / 256; ++i)
-      if (t + 256 * i < nq) Mg[(long)grp * NQ + t + 256 * i] = acc[i];
+      if (t + 256 * i < nq) st_coh(&Mg[(long)grp * NQ + t + 256 * i], acc[i]);
   }
-  __threadfence();
-  __syncthreads();
-  if (t == 0) last = (atomicAdd(&mcnt[0], 1) == ng - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+  if (!arrive_last(&mcnt[0], ng, &last)) return;
   if (t == 0) mcnt[0] = 0;
   // GV and GZ also go to LDS (Ys, Xs are free) for the correction of M
   double *gv = &Ys[0][0], *gz = &Xs[0][0];
@@ -948,7 +955,7 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
     double s = 0.0;
     if (e < nq) {
 #pragma unroll 4
-      for (int g = 0; g < ng; ++g) s += Mg[(long)g * NQ + e];
+      for (int g = 0; g < ng; ++g) s += ld_coh(&Mg[(long)g * NQ + e]);
     }
     acc[i] = s;
   }
