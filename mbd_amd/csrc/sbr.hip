@@ -451,6 +451,20 @@ __device__ inline void st_coh(double *p, double v) {
 __device__ inline double ld_coh(const double *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// maximum of v over the workgroup (every thread gets it)
+__device__ inline double block_max(double v) {
+  __shared__ double wm[NT / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = wm[0];
+#pragma unroll
+  for (int i = 1; i < NT / 64; ++i) r = fmax(r, wm[i]);
+  __syncthreads();
+  return r;
+}
+
 // true in the last of `total` workgroups to call it on cnt
 __device__ inline bool arrive_last(int *cnt, int total, int *last) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -661,7 +675,9 @@ __global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__re
   block_gram(Gs, Gp + (long)blockIdx.x * B * B);
   if (!gram_reduce_last(Gp, cnt, G)) return;
   double dev = 0.0;
-  for (int e = 0; e < B * B; ++e) dev = fmax(dev, fabs(G[e / B][e % B] - (e / B == e % B ? 1.0 : 0.0)));
+  for (int e = t; e < B * B; e += NT)
+    dev = fmax(dev, fabs(G[e / B][e % B] - (e / B == e % B ? 1.0 : 0.0)));
+  dev = block_max(dev);
   const bool ok = (dev < 0.1) && lds_chol32(G, R, 0.5);
   for (int e = t; e < B * B; e += NT) Rbuf[B * B + e] = R[e / B][e % B];
   if (t == 0) *fail = ok ? 0 : 1;
