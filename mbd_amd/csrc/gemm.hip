@@ -50,10 +50,15 @@ struct Mfma<float> {
 constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
 constexpr int PER_T = BM * BK / NT;  // elements of each operand per thread
 
+// Two waves per SIMD (launch bound): the FP64 form otherwise takes 202 VGPRs
+// plus 128 AGPRs, one workgroup per CU, and each workgroup's prologue,
+// barriers and epilogue stall the matrix cores; bounded it fits 214 VGPRs with
+// no spill, two workgroups share a CU, and the FP64 Cholesky and SYR2K updates
+// measured ~20 % faster.
 // ta: op(A)[r][k] = A[k + r lda] instead of A[r + k lda];
 // tb: op(B)[c][k] = B[k + c ldb] instead of B[c + k ldb].
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(NT) void k_gemm_nt(
+__global__ __launch_bounds__(NT, 2) void k_gemm_nt(
     int m, int n, int k, T alpha, const T *__restrict__ A, int lda, long sA,
     const T *__restrict__ B, int ldb, long sB, T beta, T *__restrict__ C,
     int ldc, long sC, int tiles_n, int ta, int tb) {
