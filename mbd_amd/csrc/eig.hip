@@ -562,14 +562,16 @@ __global__ __launch_bounds__(256) void k_gersh(int n, const double *d,
   }
 }
 
-// Multisection: 16 lanes share one eigenvalue index k and evaluate Sturm
-// counts at 16 interior points of the current interval; the group keeps the
-// sub-interval that holds eigenvalue k (17-fold reduction per sweep) until
+// Multisection: 8 lanes share one eigenvalue index k and evaluate Sturm
+// counts at 8 interior points of the current interval; the group keeps the
+// sub-interval that holds eigenvalue k (9-fold reduction per sweep) until
 // the interval is within 2 ulp (+ pivmin) of the eigenvalue.  Every
 // eigenvalue starts from one cell of an equispaced grid of kBisectGrid cells
 // over the Gershgorin interval, whose Sturm counts one pass computes for all
-// eigenvalues at once (about four sweeps fewer per eigenvalue).
-constexpr int kSect = 16;
+// eigenvalues at once (about five sweeps fewer per eigenvalue).  Eight lanes
+// measured slightly faster than sixteen (fewer counts per eigenvalue; the
+// division-bound recurrence keeps the SIMDs busy with either).
+constexpr int kSect = 8;
 constexpr int kBisectMaxSweeps = 40;
 
 __device__ inline int sturm_count(int n, const double *__restrict__ d,
@@ -631,7 +633,7 @@ __global__ __launch_bounds__(256) void k_bisect(int n, const double *__restrict_
     const double x = a + h * (double)(sl + 1);
     const bool above = (k < n) && sturm_count(n, d, e2, x, pivmin) > k;
     const unsigned long long bal = __ballot(above);
-    const unsigned bits = (unsigned)((bal >> gshift) & 0xFFFFull);
+    const unsigned bits = (unsigned)((bal >> gshift) & ((1ull << kSect) - 1));
     const int sstar = bits ? __builtin_ctz(bits) : kSect;  // first point above k
     const double na = (sstar == 0) ? a : a + h * (double)sstar;
     const double nb = (sstar == kSect) ? b : a + h * (double)(sstar + 1);
