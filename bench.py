@@ -275,6 +275,12 @@ def main():
             others[k] = roofline_entry(prof_m, k, {"chol_syrk_gemm": "fp32_tflops",
                                                    "sbr_symm": "fp64_tflops",
                                                    "trd_syr2k_gemm": "fp64_tflops"}.get(k, "hbm"))
+            r = measured_traffic_ratio(k)
+            if r is not None and prof_m[k]["bytes"] > 0:
+                others[k]["traffic"] = r * prof_m[k]["bytes"] / max(prof_m[k]["launches"], 1)
+                others[k]["traffic_basis"] = (
+                    f"counter ratio {r} (FETCH_SIZE+WRITE_SIZE over algorithmic bytes, measured at "
+                    f"n=4096) times the algorithmic bytes per launch of this run")
     out["roofline_other"] = others
     eng.close()
     if not args.no_batch:

@@ -40,4 +40,6 @@ def test_traffic_ratio_read_from_profiles():
     sys.path.insert(0, ROOT)
     import bench
     assert abs(bench.measured_traffic_ratio("trd_symv") - 1.1465) < 1e-12
+    assert abs(bench.measured_traffic_ratio("sbr_chase") - 2.2230) < 1e-12
+    assert abs(bench.measured_traffic_ratio("sbr_symm") - 2.4724) < 1e-12
     assert bench.measured_traffic_ratio("no_such_kernel") is None
