@@ -35,7 +35,13 @@ typedef struct mbd_handle mbd_handle;
 /* Solver forms. */
 #define MBD_MODE_FP64 0    /* plain FP64: tensors, Cholesky, eigen all FP64 */
 #define MBD_MODE_MIXED 1   /* FP32 tensors + FP32 MFMA Cholesky + FP64 refinement */
-#define MBD_MODE_LEARNED 2 /* MIXED tensors, network warm start + block-CG, LU fallback */
+#define MBD_MODE_LEARNED 2 /* FP32 tensors, network warm start + block-PCG */
+/* Fallback chain of a grid point's SCS solve: the block-PCG whose FP64
+ * residual fails the gate goes to the FP32 Cholesky with FP64 refinement; a
+ * point whose FP32 factor meets a non-positive pivot or whose refinement does
+ * not reach 1e-12 is solved in FP64; an FP64 Cholesky that meets a
+ * non-positive pivot (every form) is replaced by LU with partial pivoting.
+ * The reference factors with PDPOTRF (UTILS.f90:469) and ignores its info. */
 
 #define MBD_ERR_HIP 1001     /* a HIP runtime call failed */
 #define MBD_ERR_NOSYS 1002   /* mbd_set_system not called */
@@ -46,9 +52,10 @@ typedef struct mbd_handle mbd_handle;
 
 /* Diagnostics of one call (all times in milliseconds of device time). */
 typedef struct {
-  int info_scs;       /* first nonzero SCS factorisation info (leading minor
-                         index, LAPACK style); the reference ignores it */
-  int fallback_used;  /* frequencies re-solved by the fallback factorisation */
+  int info_scs;       /* first nonzero FP64 Cholesky info of an SCS grid point
+                         (leading minor index, LAPACK style); the reference
+                         ignores it, this engine solves that point by LU */
+  int fallback_used;  /* grid points re-solved by a fallback of the chain above */
   int refine_iters;   /* max iterative-refinement / Krylov iterations */
   int n_negative;     /* C-matrix eigenvalues < 0 (skipped, UTILS.f90:851) */
   int info_eig;       /* 0 ok */
@@ -179,6 +186,13 @@ int mbd_debug_cholesky(int device, int n, int batch, const double *A,
 /* All eigenvalues (ascending) of a symmetric n x n matrix (lower triangle
  * read) with the device tridiagonalisation + bisection. */
 int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w);
+
+/* X = A^-1 B (B the stacked 3x3 identity, X n x 3 per matrix) for `batch`
+ * general n x n matrices (column-major, consecutive) with the LU fallback of
+ * the SCS solve (partial pivoting, MFMA trailing updates); info[batch] = first
+ * zero pivot + 1, as LAPACK's dgetrf.  Replaces PDPOTRF/PDPOTRS at
+ * UTILS.f90:469-473 for a relay matrix that is not positive definite. */
+int mbd_debug_lu_solve(int device, int n, int batch, const double *A, double *X, int *info);
 
 /* C = alpha A B^T + beta C with the MFMA GEMM (A m x k, B n x k, C m x n,
  * column-major, leading dimensions = rows).  fp32 != 0 uses the FP32 MFMA

@@ -90,6 +90,7 @@ def lib():
     L.mbd_debug_cfdm_matrix.argtypes = [h, _D, _D, _D, _D]
     L.mbd_debug_cholesky.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _I]
     L.mbd_debug_sym_eigvals.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D]
+    L.mbd_debug_lu_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _I]
     L.mbd_debug_gemm_nt.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_double, _D, _D, ctypes.c_double, _D]
     L.mbd_batch_set.argtypes = [h, ctypes.c_int, _I, _D, _D, _D, _D, _D, ctypes.c_double,

@@ -4,10 +4,16 @@
 // Both builders tile the lower triangle of the atom (or supercell-site) pair
 // space in 16 x 16 tiles, one thread per pair.  A thread scans the lattice
 // image box with the reference's FP64 inclusion test r <= rc (computed without
-// FMA contraction, so the image set equals the reference's), sums the damped
-// tensors of the included images, and drops its 3x3 block into an LDS tile;
-// the 48 x 48 tile and its transpose are then written with coalesced
-// column-major stores, so the matrix is stored full (both triangles).
+// FMA contraction, so the image set equals the reference's; a squared-distance
+// prefilter with a rounding margin skips the square root of every image that
+// is certainly outside), sums the damped tensors of the included images, and
+// drops its 3x3 block into an LDS tile; the 48 x 48 tile is written with
+// coalesced column-major stores, and its transpose too when the caller needs
+// the full matrix (the factorisations and the eigen reduction read only the
+// lower triangle).
+// The SCS builder scans the image box once per group of kFreqGroup grid
+// points: the image set, distances, products d_a d_b and the Fermi damping do
+// not depend on the frequency; only sigma (erf and the Gaussian) does.
 //
 // Reference formulas: UTILS.f90:637-696 (SR tensor), 1098-1153 (LR tensor),
 // 543-633 (SCS blocks), 883-1094 (supercell / CFDM blocks), 1157-1184
@@ -21,6 +27,7 @@ namespace mbd {
 
 namespace {
 
+constexpr int kFreqGroup = 7;   // grid points per image scan of the SCS builder
 constexpr int TILE = 16;        // atoms (sites) per tile edge
 constexpr int TB = 3 * TILE;    // matrix rows per tile edge
 constexpr int TBP = TB + 1;     // padded LDS column
@@ -31,6 +38,24 @@ __device__ inline void tri_index(int idx, int &ti, int &tj) {
   while (t * (t + 1) / 2 > idx) --t;
   ti = t;
   tj = idx - t * (t + 1) / 2;
+}
+
+// The reference's inclusion test r <= rc with r = sqrt(dx^2 + dy^2 + dz^2)
+// rounded as exact_dist: s2 is the rounded sum of squares, rc2x the rounded
+// rc^2 times (1 + 4 eps).  s2 > rc2x implies r > rc (sqrt is monotonic and
+// correctly rounded), so the square root is taken only near or inside the
+// sphere, where r itself is needed.
+__device__ inline bool in_sphere(double dx, double dy, double dz, double rc, double rc2x,
+                                 double &r) {
+  double s2;
+  {
+#pragma clang fp contract(off)
+    s2 = dx * dx + dy * dy;
+    s2 = s2 + dz * dz;
+  }
+  if (s2 > rc2x) return false;
+  r = sqrt(s2);
+  return r <= rc;
 }
 
 // Six unique entries of a symmetric 3x3: xx, yy, zz, xy, xz, yz.
@@ -56,28 +81,6 @@ struct Comp6 {
   __device__ float get(int i) const { return s[i] + c[i]; }
 };
 
-// SR tensor in FP32 with the cancellation-free damping e/(1+e).
-__device__ inline void sr_tensor_add32(float dx, float dy, float dz, float r,
-                                       float sgm, float rvdw, float beta,
-                                       Comp6 &acc) {
-  const float fermi = r / (beta * rvdw);
-  const float x = r / sgm;
-  const float ex = __expf(-(x * x));
-  const float r2 = r * r;
-  const float zl = (erff(x) - 1.12837917f * x * ex) / (r2 * r2 * r);
-  const float zr = (2.25675833f * ex) / (sgm * sgm * sgm * r2);
-  const float ef = __expf(-6.0f * (fermi - 1.0f));
-  const float damp = isinf(ef) ? 1.0f : ef / (1.0f + ef);
-  const float d[3] = {dx, dy, dz};
-  const int rr[6] = {0, 1, 2, 0, 0, 1}, cc[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-  for (int e = 0; e < 6; ++e) {
-    const float dd = d[rr[e]] * d[cc[e]];
-    const float t = 3.0f * dd - (e < 3 ? r2 : 0.0f);
-    acc.add(e, (-t * zl + dd * zr) * damp);
-  }
-}
-
 // Write a symmetric 3x3 (6 entries) into the LDS tile at block (bi, bj).
 template <typename T>
 __device__ inline void put_block(T (*S)[TBP], int bi, int bj, const T *v6) {
@@ -91,13 +94,13 @@ __device__ inline void put_block(T (*S)[TBP], int bi, int bj, const T *v6) {
 // Store the LDS tile at (ti, tj) and, off the diagonal, its transpose.
 template <typename T>
 __device__ inline void store_tile(T (*S)[TBP], T *M, long ld, int n, int ti,
-                                  int tj) {
+                                  int tj, bool full) {
   const int r0 = TB * ti, c0 = TB * tj;
   for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
     const int r = e % TB, c = e / TB;
     if (r0 + r < n && c0 + c < n) M[(r0 + r) + (long)(c0 + c) * ld] = S[c][r];
   }
-  if (ti != tj) {
+  if (full && ti != tj) {
     for (int e = threadIdx.x; e < TB * TB; e += blockDim.x) {
       const int a = e % TB, b = e / TB;  // row c0+a, column r0+b
       if (c0 + a < n && r0 + b < n) M[(c0 + a) + (long)(r0 + b) * ld] = S[a][b];
@@ -139,114 +142,159 @@ void atom_params(hipStream_t s, int N, int nf, const double *omega,
 }
 
 // ---------------------------------------------------------------------------
-// SCS relay matrices, one frequency per blockIdx.y.
+// SCS relay matrices of nf consecutive grid points (UTILS.f90:543-633): a
+// workgroup per 16 x 16 atom-pair tile, one image scan per group of
+// kFreqGroup grid points.  Off-diagonal block (i, j): sum over images n of
+// T_sr(r_i - r_j + L_n; sqrt(s_i^2 + s_j^2), R_i + R_j); diagonal block:
+// (1/alpha_i) I + sum over n != 0 of T_sr(L_n; s_i, 2 R_i).
 template <typename T>
 __global__ __launch_bounds__(256) void k_scs_build(
     int N, const double *__restrict__ xyz, int nimg, int center,
     const double *__restrict__ img, double rc, double beta,
     const double *__restrict__ alpha_w, const double *__restrict__ sgm,
     const double *__restrict__ rvdw, T *__restrict__ A, int lda,
-    long stride) {
+    long stride, int nf, bool full) {
+  constexpr int F = kFreqGroup;
   __shared__ T S[TB][TBP];
   int ti, tj;
   tri_index(blockIdx.x, ti, tj);
-  const int f = blockIdx.y;
-  const double *aw = alpha_w + (long)f * N;
-  const double *sg = sgm + (long)f * N;
-  T *M = A + (long)f * stride;
   const int n = 3 * N;
   const int lx = threadIdx.x % TILE, ly = threadIdx.x / TILE;
   const int i = TILE * ti + lx, j = TILE * tj + ly;
   const bool active = (i < N && j < N && (ti != tj || i >= j));
+  const bool diag = (i == j);
+  const double rc2x = (rc * rc) * (1.0 + 4.0 * 2.220446049250313e-16);
+  double d0x = 0.0, d0y = 0.0, d0z = 0.0, R12 = 0.0;
   if (active) {
-    T out[6];
-    if (sizeof(T) == 8) {
-      double acc[6] = {0, 0, 0, 0, 0, 0};
-      if (i == j) {
-        const double s_i = sg[i], R2 = 2.0 * rvdw[i];
-        for (int s = 0; s < nimg; ++s) {
-          if (s == center) continue;
-          const double dx = img[3 * s], dy = img[3 * s + 1], dz = img[3 * s + 2];
-          const double r = exact_dist(dx, dy, dz);
-          if (r <= rc) sr_tensor_add64(dx, dy, dz, r, s_i, R2, beta, acc);
-        }
-        const double inv = 1.0 / aw[i];
-        acc[0] += inv;
-        acc[1] += inv;
-        acc[2] += inv;
-      } else {
-        const double d0x = xyz[3 * i] - xyz[3 * j], d0y = xyz[3 * i + 1] - xyz[3 * j + 1],
-                     d0z = xyz[3 * i + 2] - xyz[3 * j + 2];
-        const double s12 = sqrt(sg[i] * sg[i] + sg[j] * sg[j]);
-        const double R12 = rvdw[i] + rvdw[j];
-        for (int s = 0; s < nimg; ++s) {
-          double dx, dy, dz;
-          add3(d0x, d0y, d0z, img[3 * s], img[3 * s + 1], img[3 * s + 2], dx, dy, dz);
-          const double r = exact_dist(dx, dy, dz);
-          if (r <= rc) sr_tensor_add64(dx, dy, dz, r, s12, R12, beta, acc);
-        }
-      }
-      for (int e = 0; e < 6; ++e) out[e] = (T)acc[e];
+    if (diag) {
+      R12 = 2.0 * rvdw[i];
     } else {
-      Comp6 acc;
-      acc.zero();
-      const float fb = (float)beta;
-      if (i == j) {
-        const float s_i = (float)sg[i], R2 = (float)(2.0 * rvdw[i]);
-        for (int s = 0; s < nimg; ++s) {
-          if (s == center) continue;
-          const double dx = img[3 * s], dy = img[3 * s + 1], dz = img[3 * s + 2];
-          const double r = exact_dist(dx, dy, dz);
-          if (r <= rc)
-            sr_tensor_add32((float)dx, (float)dy, (float)dz, (float)r, s_i, R2, fb, acc);
+      d0x = xyz[3 * i] - xyz[3 * j];
+      d0y = xyz[3 * i + 1] - xyz[3 * j + 1];
+      d0z = xyz[3 * i + 2] - xyz[3 * j + 2];
+      R12 = rvdw[i] + rvdw[j];
+    }
+  }
+  for (int f0 = 0; f0 < nf; f0 += F) {
+    const int nfg = min(F, nf - f0);
+    T out[F][6];
+    if (active) {
+      // sigma of the pair at each grid point of the group
+      double sg12[F];
+#pragma unroll
+      for (int g = 0; g < F; ++g) {
+        const int f = f0 + min(g, nfg - 1);
+        const double si = sgm[(long)f * N + i], sj = sgm[(long)f * N + j];
+        sg12[g] = diag ? si : sqrt(si * si + sj * sj);
+      }
+      if (sizeof(T) == 8) {
+        double acc[F][6];
+#pragma unroll
+        for (int g = 0; g < F; ++g)
+#pragma unroll
+          for (int e = 0; e < 6; ++e) acc[g][e] = 0.0;
+        for (int sidx = 0; sidx < nimg; ++sidx) {
+          if (diag && sidx == center) continue;
+          double dx, dy, dz, r;
+          add3(d0x, d0y, d0z, img[3 * sidx], img[3 * sidx + 1], img[3 * sidx + 2], dx, dy, dz);
+          if (!in_sphere(dx, dy, dz, rc, rc2x, r)) continue;
+#pragma unroll
+          for (int g = 0; g < F; ++g)
+            if (g < nfg) sr_tensor_add64(dx, dy, dz, r, sg12[g], R12, beta, acc[g]);
         }
-        const double inv = 1.0 / aw[i];
-        for (int e = 0; e < 6; ++e)
-          out[e] = (T)((double)acc.s[e] + (double)acc.c[e] + (e < 3 ? inv : 0.0));
+#pragma unroll
+        for (int g = 0; g < F; ++g) {
+          const double inv = diag ? 1.0 / alpha_w[(long)(f0 + min(g, nfg - 1)) * N + i] : 0.0;
+#pragma unroll
+          for (int e = 0; e < 6; ++e) out[g][e] = (T)(acc[g][e] + (e < 3 ? inv : 0.0));
+        }
       } else {
-        const double d0x = xyz[3 * i] - xyz[3 * j], d0y = xyz[3 * i + 1] - xyz[3 * j + 1],
-                     d0z = xyz[3 * i + 2] - xyz[3 * j + 2];
-        const float s12 = (float)sqrt(sg[i] * sg[i] + sg[j] * sg[j]);
-        const float R12 = (float)(rvdw[i] + rvdw[j]);
-        for (int s = 0; s < nimg; ++s) {
-          double dx, dy, dz;
-          add3(d0x, d0y, d0z, img[3 * s], img[3 * s + 1], img[3 * s + 2], dx, dy, dz);
-          const double r = exact_dist(dx, dy, dz);
-          if (r <= rc)
-            sr_tensor_add32((float)dx, (float)dy, (float)dz, (float)r, s12, R12, fb, acc);
+        Comp6 acc[F];
+#pragma unroll
+        for (int g = 0; g < F; ++g) acc[g].zero();
+        const float fb = (float)beta, fR = (float)R12;
+        float fs[F];
+#pragma unroll
+        for (int g = 0; g < F; ++g) fs[g] = (float)sg12[g];
+        for (int sidx = 0; sidx < nimg; ++sidx) {
+          if (diag && sidx == center) continue;
+          double dx, dy, dz, r;
+          add3(d0x, d0y, d0z, img[3 * sidx], img[3 * sidx + 1], img[3 * sidx + 2], dx, dy, dz);
+          if (!in_sphere(dx, dy, dz, rc, rc2x, r)) continue;
+          // frequency-independent parts: products, powers of r, damping
+          const float x0 = (float)dx, x1 = (float)dy, x2 = (float)dz, rf = (float)r;
+          const float r2 = rf * rf;
+          const float r5 = r2 * r2 * rf;
+          const float ef = __expf(-6.0f * (rf / (fb * fR) - 1.0f));
+          const float damp = isinf(ef) ? 1.0f : ef / (1.0f + ef);
+          const float dd[6] = {x0 * x0, x1 * x1, x2 * x2, x0 * x1, x0 * x2, x1 * x2};
+#pragma unroll
+          for (int g = 0; g < F; ++g) {
+            if (g < nfg) {
+              const float sg = fs[g];
+              const float x = rf / sg;
+              const float ex = __expf(-(x * x));
+              const float zl = (erff(x) - 1.12837917f * x * ex) / r5;
+              const float zr = (2.25675833f * ex) / (sg * sg * sg * r2);
+#pragma unroll
+              for (int e = 0; e < 6; ++e) {
+                const float t = 3.0f * dd[e] - (e < 3 ? r2 : 0.0f);
+                acc[g].add(e, (-t * zl + dd[e] * zr) * damp);
+              }
+            }
+          }
         }
-        for (int e = 0; e < 6; ++e) out[e] = (T)acc.get(e);
+#pragma unroll
+        for (int g = 0; g < F; ++g) {
+          if (diag) {
+            const double inv = 1.0 / alpha_w[(long)(f0 + min(g, nfg - 1)) * N + i];
+#pragma unroll
+            for (int e = 0; e < 6; ++e)
+              out[g][e] = (T)((double)acc[g].s[e] + (double)acc[g].c[e] + (e < 3 ? inv : 0.0));
+          } else {
+#pragma unroll
+            for (int e = 0; e < 6; ++e) out[g][e] = (T)acc[g].get(e);
+          }
+        }
       }
     }
-    put_block(S, lx, ly, out);
-    if (ti == tj && i != j) put_block(S, ly, lx, out);
+#pragma unroll
+    for (int g = 0; g < F; ++g) {
+      if (g < nfg) {
+        __syncthreads();
+        if (active) {
+          put_block(S, lx, ly, out[g]);
+          if (ti == tj && !diag) put_block(S, ly, lx, out[g]);
+        }
+        __syncthreads();
+        store_tile(S, A + (long)(f0 + g) * stride, lda, n, ti, tj, full);
+      }
+    }
   }
-  __syncthreads();
-  store_tile(S, M, lda, n, ti, tj);
 }
 
 template <typename T>
 void scs_build(hipStream_t s, int N, const double *xyz, int nimg, int center,
                const double *img, double rc_bohr, double beta,
                const double *alpha_w, const double *sgm, const double *rvdw,
-               T *A, int lda, long stride, int nf) {
+               T *A, int lda, long stride, int nf, bool full) {
   const int tn = ceil_div(N, TILE);
-  dim3 grid(tn * (tn + 1) / 2, nf);
+  dim3 grid(tn * (tn + 1) / 2);
   // algorithmic output: the lower 3x3 blocks of each matrix
   ProfScope ps(K_SCS_BUILD, s, 9.0 * sizeof(T) * 0.5 * N * (N + 1.0) * nf, 0.0);
   hipLaunchKernelGGL((k_scs_build<T>), grid, dim3(256), 0, s, N, xyz, nimg,
                      center, img, rc_bohr, beta, alpha_w, sgm, rvdw, A, lda,
-                     stride);
+                     stride, nf, full);
   MBD_LAUNCH_CHECK();
 }
 template void scs_build<double>(hipStream_t, int, const double *, int, int,
                                 const double *, double, double, const double *,
                                 const double *, const double *, double *, int,
-                                long, int);
+                                long, int, bool);
 template void scs_build<float>(hipStream_t, int, const double *, int, int,
                                const double *, double, double, const double *,
                                const double *, const double *, float *, int,
-                               long, int);
+                               long, int, bool);
 
 // ---------------------------------------------------------------------------
 // MBD per-atom quantities (UTILS.f90:737-739).
@@ -280,8 +328,9 @@ __global__ __launch_bounds__(256) void k_cfdm_build(
     const int *__restrict__ cell, const double *__restrict__ lat, int nimg,
     int center, const double *__restrict__ img, double rc, double beta,
     const double *__restrict__ alpha_eff, const double *__restrict__ omega,
-    const double *__restrict__ reff, double *__restrict__ C, long ldc) {
+    const double *__restrict__ reff, double *__restrict__ C, long ldc, bool full) {
   __shared__ double S[TB][TBP];
+  const double rc2x = (rc * rc) * (1.0 + 4.0 * 2.220446049250313e-16);
   int ti, tj;
   tri_index(blockIdx.x, ti, tj);
   const int nsite = N * ncell;
@@ -300,8 +349,8 @@ __global__ __launch_bounds__(256) void k_cfdm_build(
       for (int s = 0; s < nimg; ++s) {
         if (s == center) continue;
         const double dx = img[3 * s], dy = img[3 * s + 1], dz = img[3 * s + 2];
-        const double r = exact_dist(dx, dy, dz);
-        if (r <= rc) lr_tensor_add64(dx, dy, dz, r, R2, beta, pref, acc);
+        double r;
+        if (in_sphere(dx, dy, dz, rc, rc2x, r)) lr_tensor_add64(dx, dy, dz, r, R2, beta, pref, acc);
       }
       const double w2 = omega[p] * omega[p];
       acc[0] += w2;
@@ -325,31 +374,30 @@ __global__ __launch_bounds__(256) void k_cfdm_build(
         d0z = (xyz[3 * p + 2] - xyz[3 * q + 2]) + dcz;
       }
       for (int s = 0; s < nimg; ++s) {
-        double dx, dy, dz;
+        double dx, dy, dz, r;
         add3(d0x, d0y, d0z, img[3 * s], img[3 * s + 1], img[3 * s + 2], dx, dy, dz);
-        const double r = exact_dist(dx, dy, dz);
-        if (r <= rc) lr_tensor_add64(dx, dy, dz, r, R12, beta, pref, acc);
+        if (in_sphere(dx, dy, dz, rc, rc2x, r)) lr_tensor_add64(dx, dy, dz, r, R12, beta, pref, acc);
       }
     }
     put_block(S, lx, ly, acc);
     if (ti == tj && si != sj) put_block(S, ly, lx, acc);
   }
   __syncthreads();
-  store_tile(S, C, ldc, n, ti, tj);
+  store_tile(S, C, ldc, n, ti, tj, full);
 }
 
 void cfdm_build(hipStream_t s, int N, int ncell, const double *xyz,
                 const int *cell, const double *lat, int nimg, int center,
                 const double *img, double rc_bohr, double beta,
                 const double *alpha_eff, const double *omega,
-                const double *reff, double *C, long ldc) {
+                const double *reff, double *C, long ldc, bool full) {
   const int tn = ceil_div((long)N * ncell, TILE);
   dim3 grid((unsigned)((long)tn * (tn + 1) / 2));
   const double ns = (double)N * ncell;
   ProfScope ps(K_CFDM_BUILD, s, 72.0 * 0.5 * ns * (ns + 1.0), 0.0);
   hipLaunchKernelGGL(k_cfdm_build, grid, dim3(256), 0, s, N, ncell, xyz, cell,
                      lat, nimg, center, img, rc_bohr, beta, alpha_eff, omega,
-                     reff, C, ldc);
+                     reff, C, ldc, full);
   MBD_LAUNCH_CHECK();
 }
 

@@ -168,6 +168,10 @@ struct mbd_handle {
   DevBuf<float> A32, L32, inv32;
   DevBuf<int> info;
   DevBuf<double> rnorm;
+  // FP64 re-solve of single grid points (fallback of the FP32 / iterative
+  // forms) and the LU fallback of a failed FP64 Cholesky
+  DevBuf<double> A1, inv1, luUt;
+  DevBuf<int> info1, luPiv;
   // device: iterative (learned) form
   DevBuf<double> pMinv, pR, pZ, pP, pQ, pscal, ppart, alpha_pred;
   bool have_net = false;
@@ -280,12 +284,16 @@ int freq_batch(const mbd_handle *h, size_t bytes_per_matrix) {
 
 struct DirectStats {
   int iters = 0;
-  double residual = 0.0;
+  std::vector<double> res;  // final max |B - A X| per matrix
+  std::vector<int> info;    // FP32 Cholesky info per matrix
 };
 
 // FP32 Cholesky of matrices [b0, b0 + bs) of h->A32 (copied to h->L32) and
 // the solve with FP64 iterative refinement against the FP32 tensor; X holds
-// those matrices' solutions (stride sX).
+// those matrices' solutions (stride sX).  Refinement stops once every matrix
+// meets kRefineTol or after kRefineMaxIter sweeps; the caller re-solves the
+// matrices that failed (a non-positive FP32 pivot or a residual above the
+// tolerance) in FP64.
 DirectStats solve_mixed_direct(mbd_handle *h, int n, int lda, long sA, long sinv, long sX,
                                int b0, int bs, double *X, int *info) {
   hipStream_t s = h->stream;
@@ -298,21 +306,75 @@ DirectStats solve_mixed_direct(mbd_handle *h, int n, int lda, long sA, long sinv
   set_rhs(s, n, X, n, sX, bs);
   potrs3_batched<float>(s, n, L, lda, sA, bs, inv, sinv, X, n, sX);
   DirectStats st;
+  st.res.assign(bs, 0.0);
+  st.info.assign(bs, 0);
+  MBD_HIP(hipMemcpyAsync(st.info.data(), info, sizeof(int) * bs, hipMemcpyDeviceToHost, s));
   std::vector<double> rn(2 * bs);
   for (st.iters = 0; st.iters < kRefineMaxIter; ++st.iters) {
     residual3<float>(s, n, A, lda, sA, bs, X, n, sX, R, n, sX, h->rnorm.p);
     MBD_HIP(hipMemcpyAsync(rn.data(), h->rnorm.p, sizeof(double) * 2 * bs,
                            hipMemcpyDeviceToHost, s));
     MBD_HIP(hipStreamSynchronize(s));
-    // ||b||_inf = 1 for the stacked identity, so max |r| is relative
-    double worst = 0.0;
-    for (int b = 0; b < bs; ++b) worst = std::max(worst, rn[2 * b]);
-    st.residual = worst;
-    if (worst <= kRefineTol) break;
+    // ||b||_inf = 1 for the stacked identity, so max |r| is relative; a NaN
+    // residual (broken factor) fails the test below
+    bool done = true;
+    for (int b = 0; b < bs; ++b) {
+      st.res[b] = rn[2 * b];
+      if (!(rn[2 * b] <= kRefineTol) && st.info[b] == 0) done = false;
+    }
+    if (done) break;
     potrs3_batched<float>(s, n, L, lda, sA, bs, inv, sinv, R, n, sX);
     axpy3(s, n, bs, R, n, sX, X, n, sX);
   }
   return st;
+}
+
+// Grid point k solved by LU with partial pivoting on its FP64 relay matrix
+// (rebuilt into the single-matrix buffer A1): the fallback when the FP64
+// Cholesky meets a non-positive pivot.  Xk receives A^-1 B.
+void solve_point_lu(mbd_handle *h, int k, double *Xk) {
+  const int N = h->N, n = h->n();
+  hipStream_t s = h->stream;
+  const long sA = (long)n * n;
+  h->A1.ensure((size_t)sA);
+  h->luPiv.ensure((size_t)n);
+  h->luUt.ensure((size_t)std::max(1, n - kLuNB) * kLuNB);
+  h->info1.ensure(1);
+  MBD_HIP(hipMemsetAsync(h->info1.p, 0, sizeof(int), s));
+  scs_build<double>(s, N, h->xyz.p, h->nimg_scs, h->nimg_scs / 2, h->img_scs.p, h->rc_scs / kBohr,
+                    h->beta, h->alpha_w.p + (long)k * N, h->sgm.p + (long)k * N, h->rvdw.p,
+                    h->A1.p, n, sA, 1, true);
+  getrf_batched(s, n, h->A1.p, n, sA, 1, h->luPiv.p, h->luUt.p, 0, h->info1.p);
+  set_rhs(s, n, Xk, n, 3L * n, 1);
+  getrs3_batched(s, n, h->A1.p, n, sA, 1, h->luPiv.p, Xk, n, 3L * n);
+}
+
+// Grid point k solved in FP64 (relay matrix, Cholesky, and LU when the
+// Cholesky fails): the fallback of the FP32 and iterative forms.  Returns the
+// FP64 Cholesky info (0 when it succeeded).
+int solve_point_fp64(mbd_handle *h, int k, double *Xk) {
+  const int N = h->N, n = h->n();
+  hipStream_t s = h->stream;
+  const long sA = (long)n * n;
+  const long sinv = (long)ceil_div(n, kCholNB) * kCholNB * kCholNB;
+  h->A1.ensure((size_t)sA);
+  h->inv1.ensure((size_t)sinv);
+  h->info1.ensure(1);
+  MBD_HIP(hipMemsetAsync(h->info1.p, 0, sizeof(int), s));
+  scs_build<double>(s, N, h->xyz.p, h->nimg_scs, h->nimg_scs / 2, h->img_scs.p, h->rc_scs / kBohr,
+                    h->beta, h->alpha_w.p + (long)k * N, h->sgm.p + (long)k * N, h->rvdw.p,
+                    h->A1.p, n, sA, 1, false);
+  potrf_batched<double>(s, n, h->A1.p, n, sA, 1, h->inv1.p, sinv, h->info1.p);
+  int info = 0;
+  MBD_HIP(hipMemcpyAsync(&info, h->info1.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  MBD_HIP(hipStreamSynchronize(s));
+  if (info == 0) {
+    set_rhs(s, n, Xk, n, 3L * n, 1);
+    potrs3_batched<double>(s, n, h->A1.p, n, sA, 1, h->inv1.p, sinv, Xk, n, 3L * n);
+  } else {
+    solve_point_lu(h, k, Xk);
+  }
+  return info;
 }
 
 // SCS at all 21 points into h->alpha (21 x N), h->P (21 x 9), then C6.
@@ -338,6 +400,10 @@ void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
   const int center = h->nimg_scs / 2;
   int max_it = 0;
   double max_res = 0.0;
+  // per grid point: FP64 Cholesky info where an FP64 factorisation ran (all
+  // points in the FP64 form, the fallback points otherwise)
+  std::vector<int> info64(kNumFreq, 0);
+  std::vector<char> fell(kNumFreq, 0);  // points re-solved by a fallback
   if (mode == MBD_MODE_FP64) {
     const int nb = freq_batch(h, (size_t)sA * 8 + (size_t)sinv * 8);
     h->A64.ensure((size_t)nb * sA);
@@ -346,11 +412,20 @@ void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
       const int bs = std::min(nb, kNumFreq - f0);
       scs_build<double>(s, N, h->xyz.p, h->nimg_scs, center, h->img_scs.p, rc, h->beta,
                         h->alpha_w.p + (long)f0 * N, h->sgm.p + (long)f0 * N, h->rvdw.p,
-                        h->A64.p, lda, sA, bs);
+                        h->A64.p, lda, sA, bs, false);
       potrf_batched<double>(s, n, h->A64.p, lda, sA, bs, h->inv64.p, sinv, h->info.p + f0);
       set_rhs(s, n, h->X.p + f0 * sX, n, sX, bs);
       potrs3_batched<double>(s, n, h->A64.p, lda, sA, bs, h->inv64.p, sinv,
                              h->X.p + f0 * sX, n, sX);
+      MBD_HIP(hipMemcpyAsync(info64.data() + f0, h->info.p + f0, sizeof(int) * bs,
+                             hipMemcpyDeviceToHost, s));
+      MBD_HIP(hipStreamSynchronize(s));
+      // a non-positive pivot: that point's system is solved by LU instead
+      for (int b = 0; b < bs; ++b)
+        if (info64[f0 + b] != 0) {
+          fell[f0 + b] = 1;
+          solve_point_lu(h, f0 + b, h->X.p + (f0 + b) * sX);
+        }
       contract(s, N, h->X.p + f0 * sX, n, sX, bs, h->alpha.p + (long)f0 * N, h->P.p + 9 * f0);
     }
   } else {
@@ -390,17 +465,23 @@ void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
       mlp_forward(s, (int)S, h->net_blob.p + 2 * kNetIn, h->net_Xn.p, act);
       net_alpha(s, (int)S, act[3], h->alpha_w.p, h->alpha_pred.p);
     }
-    int fallbacks = 0;
     for (int f0 = 0; f0 < kNumFreq; f0 += nb) {
       const int bs = std::min(nb, kNumFreq - f0);
       scs_build<float>(s, N, h->xyz.p, h->nimg_scs, center, h->img_scs.p, rc, h->beta,
                        h->alpha_w.p + (long)f0 * N, h->sgm.p + (long)f0 * N, h->rvdw.p,
-                       h->A32.p, lda, sA, bs);
+                       h->A32.p, lda, sA, bs, true);
       double *Xb = h->X.p + f0 * sX;
+      // matrices of this batch whose FP32 / iterative solve failed
+      std::vector<int> redo;
       if (mode == MBD_MODE_MIXED) {
         DirectStats st = solve_mixed_direct(h, n, lda, sA, sinv, sX, 0, bs, Xb, h->info.p + f0);
         max_it = std::max(max_it, st.iters);
-        max_res = std::max(max_res, st.residual);
+        for (int b = 0; b < bs; ++b) {
+          if (st.info[b] == 0 && st.res[b] <= kRefineTol)
+            max_res = std::max(max_res, st.res[b]);
+          else
+            redo.push_back(b);
+        }
       } else {
         PcgWork wk;
         wk.Minv = h->pMinv.p;
@@ -421,28 +502,35 @@ void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
             continue;
           }
           // the iterative residual failed: direct FP32 Cholesky + FP64 refinement
-          ++fallbacks;
+          fell[f0 + b] = 1;
           DirectStats st =
               solve_mixed_direct(h, n, lda, sA, sinv, sX, b, 1, Xb + b * sX, h->info.p + f0 + b);
-          max_res = std::max(max_res, st.residual);
+          if (st.info[0] == 0 && st.res[0] <= kRefineTol)
+            max_res = std::max(max_res, st.res[0]);
+          else
+            redo.push_back(b);
         }
+      }
+      // FP32 factor not positive definite or refinement not converged: the
+      // point is solved in FP64 (Cholesky, LU if that fails too)
+      for (int b : redo) {
+        fell[f0 + b] = 1;
+        info64[f0 + b] = solve_point_fp64(h, f0 + b, Xb + b * sX);
       }
       contract(s, N, Xb, n, sX, bs, h->alpha.p + (long)f0 * N, h->P.p + 9 * f0);
     }
-    if (d) d->fallback_used = fallbacks;
   }
+  if (d) d->fallback_used = (int)std::count(fell.begin(), fell.end(), 1);
   h->eig3.ensure(3 * kNumFreq);
   h->c6mol.ensure(1);
   h->c6.ensure(N);
   c6_sums(s, N, h->alpha.p, h->P.p, h->weight.p, h->eig3.p, h->c6mol.p, h->c6.p);
-  std::vector<int> inf(kNumFreq);
-  MBD_HIP(hipMemcpyAsync(inf.data(), h->info.p, sizeof(int) * kNumFreq, hipMemcpyDeviceToHost, s));
   MBD_HIP(hipStreamSynchronize(s));
   if (d) {
     d->info_scs = 0;
     for (int k = 0; k < kNumFreq; ++k)
-      if (inf[k] != 0) {
-        d->info_scs = inf[k];
+      if (info64[k] != 0) {
+        d->info_scs = info64[k];
         break;
       }
     d->refine_iters = max_it;
@@ -462,7 +550,7 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
   h->C.ensure((size_t)nm * nm);
   cfdm_build(s, N, h->ncell, h->xyz.p, h->cell.p, h->lat_d.p, h->nimg_cfdm,
              h->nimg_cfdm / 2, h->img_cfdm.p, h->rc_cfdm / kBohr, h->beta,
-             alpha_eff_d, h->om_at.p, h->reff.p, h->C.p, nm);
+             alpha_eff_d, h->om_at.p, h->reff.p, h->C.p, nm, false);
   TrdWork wk;
   h->VW.ensure((size_t)nm * 3 * kTrdNB);
   h->tau.ensure(nm);
@@ -664,6 +752,11 @@ int mbd_destroy(mbd_handle *h) {
   h->net_cnt.release();
   h->inv32.release();
   h->info.release();
+  h->A1.release();
+  h->inv1.release();
+  h->luUt.release();
+  h->info1.release();
+  h->luPiv.release();
   h->cell.release();
   h->nneg.release();
   DevBuf<double> *sbufs[] = {&h->sbrV, &h->sbrT, &h->sbrR, &h->sbrX, &h->sbrY, &h->sbrMp,
@@ -1178,14 +1271,14 @@ int mbd_debug_relay_matrix(mbd_handle *h, int mode, int k, double *A) {
       h->A64.ensure(cnt);
       scs_build<double>(s, N, h->xyz.p, h->nimg_scs, h->nimg_scs / 2, h->img_scs.p, rc,
                         h->beta, h->alpha_w.p + (long)k * N, h->sgm.p + (long)k * N,
-                        h->rvdw.p, h->A64.p, n, (long)cnt, 1);
+                        h->rvdw.p, h->A64.p, n, (long)cnt, 1, true);
       fetch(h, A, h->A64.p, cnt);
       MBD_HIP(hipStreamSynchronize(s));
     } else {
       h->A32.ensure(cnt);
       scs_build<float>(s, N, h->xyz.p, h->nimg_scs, h->nimg_scs / 2, h->img_scs.p, rc,
                        h->beta, h->alpha_w.p + (long)k * N, h->sgm.p + (long)k * N,
-                       h->rvdw.p, h->A32.p, n, (long)cnt, 1);
+                       h->rvdw.p, h->A32.p, n, (long)cnt, 1, true);
       std::vector<float> tmp(cnt);
       MBD_HIP(hipMemcpyAsync(tmp.data(), h->A32.p, sizeof(float) * cnt, hipMemcpyDeviceToHost, s));
       MBD_HIP(hipStreamSynchronize(s));
@@ -1216,7 +1309,7 @@ int mbd_debug_cfdm_matrix(mbd_handle *h, const double *alpha_eff, const double *
     h->C.ensure((size_t)nm * nm);
     cfdm_build(s, N, h->ncell, h->xyz.p, h->cell.p, h->lat_d.p, h->nimg_cfdm, h->nimg_cfdm / 2,
                h->img_cfdm.p, h->rc_cfdm / kBohr, h->beta, h->ext_alpha.p, h->om_at.p,
-               h->reff.p, h->C.p, nm);
+               h->reff.p, h->C.p, nm, true);
     fetch(h, C, h->C.p, (size_t)nm * nm);
     MBD_HIP(hipStreamSynchronize(s));
   } catch (const HipError &e) {
@@ -1339,6 +1432,35 @@ int mbd_debug_sym_eigvals(int device, int n, const double *A, double *w) {
     return fail(nullptr, e);
   }
   release();
+  return 0;
+}
+
+int mbd_debug_lu_solve(int device, int n, int batch, const double *A, double *X, int *info) {
+  if (n <= 0) return -2;
+  if (batch <= 0) return -3;
+  if (!A) return -4;
+  if (!X) return -5;
+  if (!info) return -6;
+  try {
+    MBD_HIP(hipSetDevice(device));
+    DevScratch sc;
+    const long sA = (long)n * n, sX = 3L * n, sUt = (long)std::max(1, n - kLuNB) * kLuNB;
+    double *dA = sc.get<double>((size_t)batch * sA);
+    double *dX = sc.get<double>((size_t)batch * sX);
+    double *dUt = sc.get<double>((size_t)batch * sUt);
+    int *piv = sc.get<int>((size_t)batch * n);
+    int *inf = sc.get<int>((size_t)batch);
+    MBD_HIP(hipMemcpy(dA, A, sizeof(double) * batch * sA, hipMemcpyHostToDevice));
+    MBD_HIP(hipMemset(inf, 0, sizeof(int) * batch));
+    getrf_batched(nullptr, n, dA, n, sA, batch, piv, dUt, sUt, inf);
+    set_rhs(nullptr, n, dX, n, sX, batch);
+    getrs3_batched(nullptr, n, dA, n, sA, batch, piv, dX, n, sX);
+    MBD_HIP(hipDeviceSynchronize());
+    MBD_HIP(hipMemcpy(X, dX, sizeof(double) * batch * sX, hipMemcpyDeviceToHost));
+    MBD_HIP(hipMemcpy(info, inf, sizeof(int) * batch, hipMemcpyDeviceToHost));
+  } catch (const HipError &e) {
+    return fail(nullptr, e);
+  }
   return 0;
 }
 

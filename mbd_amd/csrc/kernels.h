@@ -18,7 +18,7 @@ template <typename T>
 void scs_build(hipStream_t s, int N, const double *xyz, int nimg, int center,
                const double *img, double rc_bohr, double beta,
                const double *alpha_w, const double *sgm, const double *rvdw,
-               T *A, int lda, long stride, int nf);
+               T *A, int lda, long stride, int nf, bool full);
 void mbd_params(hipStream_t s, int N, const double *alpha_eff,
                 const double *c6_eff, const double *a0, const double *r0,
                 double *omega, double *reff);
@@ -26,7 +26,7 @@ void cfdm_build(hipStream_t s, int N, int ncell, const double *xyz,
                 const int *cell, const double *lat, int nimg, int center,
                 const double *img, double rc_bohr, double beta,
                 const double *alpha_eff, const double *omega,
-                const double *reff, double *C, long ldc);
+                const double *reff, double *C, long ldc, bool full);
 
 // chol.hip
 template <typename T>
@@ -41,6 +41,14 @@ void contract(hipStream_t s, int N, const double *X, int ldx, long sX,
               int batch, double *alpha, double *P);
 void c6_sums(hipStream_t s, int N, const double *alpha, const double *P,
              const double *weight, double *eig, double *c6_mol, double *c6);
+
+// lu.hip: LU with partial pivoting, the fallback of a failed SCS Cholesky
+constexpr int kLuNB = 32;  // panel width
+// ipiv: batch x n; Ut: per matrix (n - kLuNB) x kLuNB doubles (stride sUt)
+void getrf_batched(hipStream_t s, int n, double *A, int lda, long sA, int batch, int *ipiv,
+                   double *Ut, long sUt, int *info);
+void getrs3_batched(hipStream_t s, int n, const double *A, int lda, long sA, int batch,
+                    const int *ipiv, double *X, int ldx, long sX);
 
 // eig.hip
 // partial-dot quantities of one step (4 per reflector) and the reduced-dot

@@ -31,6 +31,7 @@ enum KernelClass {
   K_SBR_QR,
   K_SBR_SYMM,
   K_SBR_CHASE,
+  K_LU,
   K_NUM_CLASSES
 };
 

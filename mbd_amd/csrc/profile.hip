@@ -9,7 +9,8 @@ const char *kernel_class_name(int cls) {
   static const char *names[K_NUM_CLASSES] = {
       "scs_build", "chol_diag", "chol_trsm_gemm", "chol_syrk_gemm", "trsv",
       "refine_residual", "cfdm_build", "trd_colupd", "trd_symv", "trd_wcol", "trd_syr2k_gemm",
-      "trd_tail", "bisect", "pcg_symv", "pcg_small", "net_infer", "sbr_panel_qr", "sbr_symm", "sbr_chase"};
+      "trd_tail", "bisect", "pcg_symv", "pcg_small", "net_infer", "sbr_panel_qr", "sbr_symm", "sbr_chase",
+      "lu_fallback"};
   return (cls >= 0 && cls < K_NUM_CLASSES) ? names[cls] : "";
 }
 

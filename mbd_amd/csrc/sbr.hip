@@ -1080,33 +1080,133 @@ __device__ inline double half_sum16(double (&q)[16], int lane) {
   return q[0] + __shfl_xor(q[0], 1, 64);
 }
 
-// Band elements move between workgroups on different XCDs, whose L2 caches are
-// not coherent with each other: every band access is an agent-scope atomic
-// (coherent at the device level, no L2 write-back or invalidate needed), and a
-// task's completion is published only after all its stores have completed.
-__device__ inline double ld_band(const double *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_band(double *p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Memory policies of the bulge chase.  Band elements, progress counters and
+// mailbox slots move between workgroups, so every access is coherent:
+//  * ChaseAgent: the workgroups may sit on any XCD, whose L2 caches are not
+//    coherent with each other; every access is an agent-scope atomic (it goes
+//    past the XCD's L2 to the device's coherence point);
+//  * ChaseXcc: only the workgroups on one XCD take part (the XCD of the first
+//    workgroup to arrive; the others exit at once).  That XCD's L2 is then the
+//    coherence point of every participant: loads bypass the CU's L1 (the sc0
+//    cache policy: coherent across the CUs that share an L2) and hit the L2,
+//    stores write through the L1 into it.  Far shorter round trips than the
+//    device-scope accesses, which is what the sweep chain is made of.
+// In both, a task's completion is published only after all its stores have
+// been acknowledged (s_waitcnt), and a consumer issues its loads only after it
+// has read the published progress.
+struct ChaseArgs {
+  double *AB;
+  int *prog;
+  unsigned long long *mail;
+  int *flag;
+  long n_ab, n_prog, n_mail;  // element counts
+};
+
+struct ChaseAgent {
+  double *AB;
+  int *prog;
+  unsigned long long *mail;
+  int *flag;
+  __device__ explicit ChaseAgent(const ChaseArgs &a)
+      : AB(a.AB), prog(a.prog), mail(a.mail), flag(a.flag) {}
+  __device__ bool init(bool) { return true; }
+  __device__ double ld(long e) const {
+    return __hip_atomic_load(&AB[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ void st(long e, double v) const {
+    __hip_atomic_store(&AB[e], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ int ld_prog(int s) const {
+    return __hip_atomic_load(&prog[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ void st_prog(int s, int v) const {
+    __hip_atomic_store(&prog[s], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ unsigned long long ld_mail(long e) const {
+    return __hip_atomic_load(&mail[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ void st_mail(long e, unsigned long long v) const {
+    __hip_atomic_store(&mail[e], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ int ld_abort() const {
+    return __hip_atomic_load(&flag[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ void st_abort() const {
+    __hip_atomic_store(&flag[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+};
+
+// cache policy bits of the buffer intrinsics: sc0 (coherent at the L2 shared
+// by the CUs of one XCD) and the compiler's volatile flag (no hoisting or
+// merging of a polled load)
+constexpr int kSc0 = 1;
+constexpr int kSc0Volatile = 1 | (1 << 31);
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+struct ChaseXcc {
+  int *flag;
+  __amdgpu_buffer_rsrc_t rab, rprog, rmail, rflag;
+  __device__ explicit ChaseXcc(const ChaseArgs &a)
+      : flag(a.flag),
+        rab(__builtin_amdgcn_make_buffer_rsrc(a.AB, 0, (int)(a.n_ab * 8), 0x00020000)),
+        rprog(__builtin_amdgcn_make_buffer_rsrc(a.prog, 0, (int)(a.n_prog * 4), 0x00020000)),
+        rmail(__builtin_amdgcn_make_buffer_rsrc(a.mail, 0, (int)(a.n_mail * 8), 0x00020000)),
+        rflag(__builtin_amdgcn_make_buffer_rsrc(a.flag, 0, 16, 0x00020000)) {}
+  // true when this workgroup takes part: its XCD is the one that the first
+  // workgroup to arrive claimed (flag[1], -1 before the launch)
+  __device__ bool init(bool lead) {
+    int mine = 0;
+    if (lead) {
+      const int xcc = (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15);
+      int expect = -1;
+      __hip_atomic_compare_exchange_strong(&flag[1], &expect, xcc, __ATOMIC_RELAXED,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      mine = (expect == -1 || expect == xcc) ? 1 : 0;
+    }
+    return __shfl(mine, 0, 64) != 0;
+  }
+  __device__ double ld(long e) const {
+    return __builtin_bit_cast(double,
+                              __builtin_amdgcn_raw_buffer_load_b64(rab, (int)(e * 8), 0, kSc0));
+  }
+  __device__ void st(long e, double v) const {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rab, (int)(e * 8), 0, 0);
+  }
+  __device__ int ld_prog(int s) const {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(rprog, s * 4, 0, kSc0Volatile);
+  }
+  __device__ void st_prog(int s, int v) const {
+    __builtin_amdgcn_raw_buffer_store_b32((unsigned)v, rprog, s * 4, 0, 0);
+  }
+  __device__ unsigned long long ld_mail(long e) const {
+    return __builtin_bit_cast(unsigned long long,
+                              __builtin_amdgcn_raw_buffer_load_b64(rmail, (int)(e * 8), 0,
+                                                                   kSc0Volatile));
+  }
+  __device__ void st_mail(long e, unsigned long long v) const {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rmail, (int)(e * 8), 0, 0);
+  }
+  __device__ int ld_abort() const {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(rflag, 0, 0, kSc0Volatile);
+  }
+  __device__ void st_abort() const { __builtin_amdgcn_raw_buffer_store_b32(1u, rflag, 0, 0, 0); }
+};
 
 // Empty mailbox slot: a signalling NaN, which no arithmetic result can be.
 constexpr unsigned long long kMailEmpty = 0x7FF4000000000BADull;
 
-// Polls until *p >= need.  Gives up (and raises the abort flag for every other
-// sweep) after kChaseSpin polls, or when another sweep gave up.
-__device__ inline bool chase_wait(const int *p, int need, int *flag) {
+// Polls until prog[s] >= need.  Gives up (and raises the abort flag for every
+// other sweep) after kChaseSpin polls, or when another sweep gave up.
+template <class M>
+__device__ inline bool chase_wait(const M &mem, int s, int need) {
   int it = 0;
   for (;;) {
-    const int v = __builtin_amdgcn_readfirstlane(
-        __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const int v = __builtin_amdgcn_readfirstlane(mem.ld_prog(s));
     if (v >= need) return true;
-    const int ab = __builtin_amdgcn_readfirstlane(
-        __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const int ab = __builtin_amdgcn_readfirstlane(mem.ld_abort());
     if (ab != 0) return false;
     if (++it > kChaseSpin) {
-      __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      mem.st_abort();
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
@@ -1114,19 +1214,19 @@ __device__ inline bool chase_wait(const int *p, int need, int *flag) {
 }
 
 // Polls a mailbox slot until it holds a value (same limits as chase_wait).
-__device__ inline bool mail_wait(const unsigned long long *p, double &v, int *flag) {
+template <class M>
+__device__ inline bool mail_wait(const M &mem, long e, double &v) {
   int it = 0;
   for (;;) {
-    const unsigned long long u = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long u = mem.ld_mail(e);
     if (u != kMailEmpty) {
       v = __longlong_as_double((long long)u);
       return true;
     }
-    const int ab = __builtin_amdgcn_readfirstlane(
-        __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const int ab = __builtin_amdgcn_readfirstlane(mem.ld_abort());
     if (ab != 0) return false;
     if (++it > kChaseSpin) {
-      __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      mem.st_abort();
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
@@ -1135,8 +1235,8 @@ __device__ inline bool mail_wait(const unsigned long long *p, double &v, int *fl
 }  // namespace
 
 // Band storage AB[c * LDB + d] = A[c + d][c] for d <= B (zero for d > B and
-// past the matrix); clears the sweep progress counters, the abort flag and
-// the mailboxes.
+// past the matrix); clears the sweep progress counters, the abort flag, the
+// sweep claim counter and the mailboxes, and marks the chase's XCD unclaimed.
 __global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restrict__ A, long lda,
                                                   double *__restrict__ AB, int *prog, int *flag,
                                                   unsigned long long *mail, long nmail) {
@@ -1148,11 +1248,13 @@ __global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restric
     }
     if (e < nmail) mail[e] = kMailEmpty;
     if (e < n) prog[e] = 0;
-    if (e < 4) flag[e] = 0;
+    if (e < 4) flag[e] = (e == 1) ? -1 : 0;
   }
 }
 
-// Bulge chasing, one wavefront per sweep (sweeps s = blockIdx.x + k gridDim.x).
+// Bulge chasing, one wavefront per sweep; a workgroup claims sweeps in
+// increasing order from a counter (flag[2]), so every sweep it waits for has
+// been claimed by a running workgroup.
 // Task k of sweep s works on rows I = [p, p + B), p = s + 1 + k B: the
 // reflector from the first column of L = A[I, previous block] (k = 0: column
 // s), applied from the left to L, from both sides to D = A[I, I] and from the
@@ -1163,16 +1265,26 @@ __global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restric
 // k + 1; the one element it shares with task k + 2 of s - 1 (that task's
 // beta, the head of its L) arrives through a mailbox as soon as s - 1 has
 // formed the reflector, and task k + 2 leaves that element to sweep s.
-__global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB, int *prog,
-                                                  unsigned long long *mail, int mt, int *flag) {
+template <class M>
+__global__ __launch_bounds__(64) void k_sbr_chase(int n, ChaseArgs args, int mt) {
   __shared__ double vs[B], ys[B], us[B];
+  __shared__ int claim;
   const int lane = threadIdx.x, c = lane & 31, h = lane >> 5;
+  M mem(args);
+  if (!mem.init(lane == 0)) return;
   double L[16], D[16], Bn[16];
-  for (int s = blockIdx.x; s <= n - 3; s += gridDim.x) {
+  for (;;) {
+    if (lane == 0)
+      claim = __hip_atomic_fetch_add(&mem.flag[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int s = claim;
+    __syncthreads();
+    if (s > n - 3) return;
     const int nt = (n - 3 - s) / B + 1;
     const int ntp = (s > 0) ? (n - 2 - s) / B + 1 : 0;
     for (int k = 0; k < nt; ++k) {
-      if (s > 0 && !chase_wait(&prog[s - 1], min(k + 2, ntp), flag)) return;
+      if (s > 0 && !chase_wait(mem, s - 1, min(k + 2, ntp))) return;
+      asm volatile("" ::: "memory");
       const int p = s + 1 + B * k;
       const int l = min(B, n - p);
       const int l2 = max(0, min(B, n - p - B));
@@ -1180,7 +1292,7 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           const int r = 16 * h + i;
-          L[i] = (c == 0 && r < l) ? ld_band(&AB[(long)s * LDB + 1 + r]) : 0.0;
+          L[i] = (c == 0 && r < l) ? mem.ld((long)s * LDB + 1 + r) : 0.0;
         }
       }
       // |x|^2 below the head of column 0 of L; consumed before the D / Bn
@@ -1195,18 +1307,17 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const int r = 16 * h + i;
-        D[i] = (r < l && c < l) ? (r >= c ? ld_band(&AB[(long)(p + c) * LDB + (r - c)])
-                                          : ld_band(&AB[(long)(p + r) * LDB + (c - r)]))
+        D[i] = (r < l && c < l) ? (r >= c ? mem.ld((long)(p + c) * LDB + (r - c))
+                                          : mem.ld((long)(p + r) * LDB + (c - r)))
                                 : 0.0;
-        Bn[i] = (r < l2 && c < l) ? ld_band(&AB[(long)(p + c) * LDB + B + r - c]) : 0.0;
+        Bn[i] = (r < l2 && c < l) ? mem.ld((long)(p + c) * LDB + B + r - c) : 0.0;
       }
       const double xn2 = __shfl(part, 0, 64) + __shfl(part, 32, 64);
       const double alp = __shfl(L[0], 0, 64);
       double tau, beta, sc;
       reflector(xn2, alp, tau, beta, sc);
       if (k >= 2 && lane == 0)
-        __hip_atomic_store(&mail[(long)s * mt + k], (unsigned long long)__double_as_longlong(beta),
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mem.st_mail((long)s * mt + k, (unsigned long long)__double_as_longlong(beta));
       if (c == 0) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -1233,7 +1344,7 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB
       // the head of task k + 2's L of sweep s - 1 (last row, last column of Bn)
       if (s > 0 && k + 2 < ntp) {
         double bm = 0.0;
-        if (!mail_wait(&mail[(long)(s - 1) * mt + k + 2], bm, flag)) return;
+        if (!mail_wait(mem, (long)(s - 1) * mt + k + 2, bm)) return;
         if (lane == 63) Bn[15] = bm;
       }
       // y = tau D v as column sums (D is exactly symmetric in registers: both
@@ -1270,7 +1381,7 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB
 #pragma unroll
           for (int i = 0; i < 16; ++i) {
             const int r = 16 * h + i;
-            if (r < l) st_band(&AB[(long)s * LDB + 1 + r], L[i]);
+            if (r < l) mem.st((long)s * LDB + 1 + r, L[i]);
           }
         }
       } else {
@@ -1278,26 +1389,27 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, double *__restrict__ AB
         for (int i = 0; i < 16; ++i) {
           const int r = 16 * h + i;
           if (r < l && !(k >= 2 && r == 0 && c == 0))
-            st_band(&AB[(long)(p - B + c) * LDB + B + r - c], L[i]);
+            mem.st((long)(p - B + c) * LDB + B + r - c, L[i]);
         }
       }
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const int r = 16 * h + i;
-        if (r < l && c < l && r >= c) st_band(&AB[(long)(p + c) * LDB + (r - c)], D[i]);
+        if (r < l && c < l && r >= c) mem.st((long)(p + c) * LDB + (r - c), D[i]);
       }
       if (k + 1 == nt && l2 > 0) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
           const int r = 16 * h + i;
-          if (r < l2 && c < l) st_band(&AB[(long)(p + c) * LDB + B + r - c], Bn[i]);
+          if (r < l2 && c < l) mem.st((long)(p + c) * LDB + B + r - c, Bn[i]);
         }
       }
       // publish task k once every lane's stores have completed
       __syncthreads();
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_s_waitcnt(0);
-      if (lane == 0) __hip_atomic_store(&prog[s], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      asm volatile("" ::: "memory");
+      if (lane == 0) mem.st_prog(s, k + 1);
 #pragma unroll
       for (int i = 0; i < 16; ++i) L[i] = Bn[i];
     }
@@ -1336,6 +1448,13 @@ SbrSizes sbr_sizes(int n) {
   z.cq = (size_t)n / B + 4;  // two arrival counters, then one path flag per panel
   z.cqR = 2 * (size_t)B * B;
   return z;
+}
+
+// MBD_SBR_CHASE_XCC=0 runs the bulge chase on every XCD with device-scope
+// accesses instead of on one XCD through its L2
+bool sbr_chase_xcc() {
+  const char *e = getenv("MBD_SBR_CHASE_XCC");
+  return !(e && e[0] == '0');
 }
 
 bool sbr_lookahead() {
@@ -1485,8 +1604,16 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
     // about n^2 / (2B) tasks, each loading two B x B blocks and storing one and
     // a half (3.5 B^2 doubles) and doing about 12 B^2 flops
     ProfScope ps(K_SBR_CHASE, s, 8.0 * 3.5 * B * 0.5 * (double)n * n, 12.0 * B * 0.5 * (double)n * n);
-    hipLaunchKernelGGL(k_sbr_chase, dim3(std::min(n - 2, kSbrChaseWG)), dim3(64), 0, s, n, w.AB,
-                       w.prog, w.mail, sbr_mail_tasks(n), w.flag);
+    const ChaseArgs ca{w.AB, w.prog, w.mail, w.flag, (long)zz.AB, (long)zz.prog, (long)zz.mail};
+    if (sbr_chase_xcc()) {
+      // 8x the workgroups (every XCD gets a share); those outside the chosen
+      // XCD exit at once
+      hipLaunchKernelGGL(k_sbr_chase<ChaseXcc>, dim3(8 * std::min(n - 2, kSbrChaseWG)), dim3(64),
+                         0, s, n, ca, sbr_mail_tasks(n));
+    } else {
+      hipLaunchKernelGGL(k_sbr_chase<ChaseAgent>, dim3(std::min(n - 2, kSbrChaseWG)), dim3(64),
+                         0, s, n, ca, sbr_mail_tasks(n));
+    }
     MBD_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_sbr_de, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w.AB, w.d, w.e);

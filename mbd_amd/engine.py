@@ -322,6 +322,22 @@ def device_cholesky(A_batch: np.ndarray, device: int = 0):
     return np.tril(np.transpose(L, (0, 2, 1))), info
 
 
+def device_lu_solve(A_batch: np.ndarray, device: int = 0):
+    """X = A^-1 [I; I; ...] (n x 3) for a (batch, n, n) stack with the device
+    LU (partial pivoting); returns (X (batch, n, 3), info (batch,))."""
+    A = np.asarray(A_batch, dtype=np.float64)
+    if A.ndim == 2:
+        A = A[None]
+    b, n, _ = A.shape
+    src = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))  # column-major per matrix
+    X = np.zeros((b, 3, n))
+    info = np.zeros(b, dtype=np.int32)
+    rc = _lib.lib().mbd_debug_lu_solve(device, n, b, dptr(src), dptr(X), iptr(info))
+    if rc != 0:
+        raise MbdError(f"mbd_debug_lu_solve failed ({rc})")
+    return np.transpose(X, (0, 2, 1)), info
+
+
 def device_sym_eigvals(A: np.ndarray, device: int = 0) -> np.ndarray:
     n = A.shape[0]
     src = np.asfortranarray(np.asarray(A, dtype=np.float64))

@@ -49,6 +49,18 @@ def _small_sheet():
     return generators.hex_sheet(6, 4, seed=3, rc_scs=30.0)
 
 
+def _xe_chain():
+    # one Xe atom per 2.3 A along x: the self-image terms of the relay
+    # matrix's diagonal block (sigma_i, not sqrt(2) sigma_i) screen alpha_eff
+    # up to ~1200 bohr^3; the relay matrices stay positive definite, but the
+    # coupled-QHO matrix of the 5 x 1 x 1 supercell has one negative
+    # eigenvalue, which the energy skips and counts (UTILS.f90:851-863)
+    st = io.Settings(mbd_scs_dip_cutoff=20.0, mbd_supercell_cutoff=10.0,
+                     mbd_cfdm_dip_cutoff=20.0)
+    return io.Geometry(["Xe"], np.zeros((1, 3)), np.array([0.9]),
+                       np.diag([2.3, 30.0, 30.0])), st
+
+
 CASES = {
     "C6H6": lambda: _load("C6H6.in"),
     "H2O": lambda: _load("H2O.in"),
@@ -57,6 +69,7 @@ CASES = {
     "sheet48_vacuum_z": _small_sheet,
     "random64_periodic": _small_periodic,
     "ring": generators.ring,
+    "xe_chain_nimag": _xe_chain,
 }
 
 
@@ -126,6 +139,19 @@ def test_full_run_fp64(eng, case):
     assert np.allclose(got.molpol_eig, want.molpol_eig, rtol=1e-10)
     assert got.diag["supercell"] == list(want.supercell)
     assert got.diag["n_negative"] == want.nimag
+
+
+def test_negative_eigenvalue_counted_like_oracle(eng):
+    geo, st = CASES["xe_chain_nimag"]()
+    eng.set_geometry(geo, st)
+    want = O.mbd_run(_oracle_system(geo, st))
+    assert want.info == 0 and want.nimag >= 1
+    for mode, tol in (("fp64", 1e-8), ("mixed", 1e-6), ("learned", 1e-6)):
+        got = eng.run(mode)
+        print(f"xe chain {mode}: E={got.e_mbd:.12f} oracle={want.e_mbd:.12f} "
+              f"n_negative={got.diag['n_negative']} oracle NIMAG={want.nimag}", flush=True)
+        assert got.diag["n_negative"] == want.nimag
+        assert abs(got.e_mbd - want.e_mbd) <= tol
 
 
 @pytest.mark.parametrize("name", ["C6H6", "graphene_111", "PBBA"])
