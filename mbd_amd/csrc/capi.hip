@@ -172,8 +172,13 @@ struct mbd_handle {
   // forms) and the LU fallback of a failed FP64 Cholesky
   DevBuf<double> A1, inv1, luUt;
   DevBuf<int> info1, luPiv;
-  // device: iterative (learned) form
+  // device: iterative (learned) form; preconditioner clusters of the system
   DevBuf<double> pMinv, pR, pZ, pP, pQ, pscal, ppart, alpha_pred;
+  Clusters cl;
+  DevBuf<int> clOff, clAtoms;
+  DevBuf<long> clMoff;
+  // partials of the lower-triangle products (refinement residuals, block-PCG)
+  DevBuf<double> svU, svV;
   bool have_net = false;
   DevBuf<float> net_blob, net_X, net_Xn, net_act;
   DevBuf<float4> net_nbl;
@@ -311,7 +316,8 @@ DirectStats solve_mixed_direct(mbd_handle *h, int n, int lda, long sA, long sinv
   MBD_HIP(hipMemcpyAsync(st.info.data(), info, sizeof(int) * bs, hipMemcpyDeviceToHost, s));
   std::vector<double> rn(2 * bs);
   for (st.iters = 0; st.iters < kRefineMaxIter; ++st.iters) {
-    residual3<float>(s, n, A, lda, sA, bs, X, n, sX, R, n, sX, h->rnorm.p);
+    residual3<float>(s, n, A, lda, sA, bs, X, n, sX, R, n, sX, h->rnorm.p,
+                     Symv3Work{h->svU.p, h->svV.p});
     MBD_HIP(hipMemcpyAsync(rn.data(), h->rnorm.p, sizeof(double) * 2 * bs,
                            hipMemcpyDeviceToHost, s));
     MBD_HIP(hipStreamSynchronize(s));
@@ -435,9 +441,12 @@ void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
     h->inv32.ensure((size_t)nb * sinv);
     h->R.ensure((size_t)nb * sX);
     h->rnorm.ensure((size_t)2 * kNumFreq);
+    const Symv3Sizes svz = symv3_sizes(n);
+    h->svU.ensure((size_t)nb * svz.upart);
+    h->svV.ensure((size_t)nb * svz.vpart);
+    const int npart = std::max(h->cl.n, ceil_div(n, 256));
     if (mode == MBD_MODE_LEARNED) {
-      const size_t npart = (size_t)std::max(ceil_div(N, 256), ceil_div(n, 16));
-      h->pMinv.ensure((size_t)nb * N * 9);
+      h->pMinv.ensure((size_t)nb * h->cl.mtot);
       h->pR.ensure((size_t)nb * sX);
       h->pZ.ensure((size_t)nb * sX);
       h->pP.ensure((size_t)nb * sX);
@@ -469,7 +478,7 @@ void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
       const int bs = std::min(nb, kNumFreq - f0);
       scs_build<float>(s, N, h->xyz.p, h->nimg_scs, center, h->img_scs.p, rc, h->beta,
                        h->alpha_w.p + (long)f0 * N, h->sgm.p + (long)f0 * N, h->rvdw.p,
-                       h->A32.p, lda, sA, bs, true);
+                       h->A32.p, lda, sA, bs, false);
       double *Xb = h->X.p + f0 * sX;
       // matrices of this batch whose FP32 / iterative solve failed
       std::vector<int> redo;
@@ -484,6 +493,11 @@ void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
         }
       } else {
         PcgWork wk;
+        wk.cl_off = h->clOff.p;
+        wk.cl_atoms = h->clAtoms.p;
+        wk.cl_moff = h->clMoff.p;
+        wk.ncl = h->cl.n;
+        wk.cl_mtot = h->cl.mtot;
         wk.Minv = h->pMinv.p;
         wk.R = h->pR.p;
         wk.Z = h->pZ.p;
@@ -491,9 +505,11 @@ void run_scs(mbd_handle *h, int mode, mbd_diag *d) {
         wk.Q = h->pQ.p;
         wk.scal = h->pscal.p;
         wk.part = h->ppart.p;
+        wk.npart = npart;
         wk.rnorm = h->rnorm.p;
+        wk.sv = Symv3Work{h->svU.p, h->svV.p};
         const double *a0 = h->have_net ? h->alpha_pred.p + (long)f0 * N : nullptr;
-        PcgResult pr = pcg_solve<float>(s, N, h->A32.p, lda, sA, bs, a0, Xb, wk, kRefineTol,
+        PcgResult pr = pcg_solve<float>(s, N, h->A32.p, lda, sA, bs, a0, Xb, wk, kPcgTol,
                                         pcg_maxit());
         for (int b = 0; b < bs; ++b) {
           max_it = std::max(max_it, pr.iters[b]);
@@ -754,6 +770,11 @@ int mbd_destroy(mbd_handle *h) {
   h->info.release();
   h->A1.release();
   h->inv1.release();
+  h->svU.release();
+  h->svV.release();
+  h->clOff.release();
+  h->clAtoms.release();
+  h->clMoff.release();
   h->luUt.release();
   h->info1.release();
   h->luPiv.release();
@@ -899,6 +920,17 @@ int mbd_set_system(mbd_handle *h, int n_atoms, const double *xyz, const double *
     }
     up(h->img_cfdm, tc.data(), tc.size());
     up(h->lat_d, h->lat, 9);
+    // atom clusters of the iterative solve's block preconditioner
+    h->cl = build_clusters(n_atoms, xyz, h->lat, h->periodic != 0, hirshfeld, alpha0, r0, beta);
+    h->clOff.ensure(h->cl.off.size());
+    h->clAtoms.ensure(h->cl.atoms.size());
+    h->clMoff.ensure(h->cl.moff.size());
+    MBD_HIP(hipMemcpyAsync(h->clOff.p, h->cl.off.data(), sizeof(int) * h->cl.off.size(),
+                           hipMemcpyHostToDevice, h->stream));
+    MBD_HIP(hipMemcpyAsync(h->clAtoms.p, h->cl.atoms.data(), sizeof(int) * h->cl.atoms.size(),
+                           hipMemcpyHostToDevice, h->stream));
+    MBD_HIP(hipMemcpyAsync(h->clMoff.p, h->cl.moff.data(), sizeof(long) * h->cl.moff.size(),
+                           hipMemcpyHostToDevice, h->stream));
     h->cell.ensure(cells.size());
     MBD_HIP(hipMemcpyAsync(h->cell.p, cells.data(), sizeof(int) * cells.size(),
                            hipMemcpyHostToDevice, h->stream));

@@ -1080,26 +1080,18 @@ __device__ inline double half_sum16(double (&q)[16], int lane) {
   return q[0] + __shfl_xor(q[0], 1, 64);
 }
 
-// Memory policies of the bulge chase.  Band elements, progress counters and
-// mailbox slots move between workgroups, so every access is coherent:
-//  * ChaseAgent: the workgroups may sit on any XCD, whose L2 caches are not
-//    coherent with each other; every access is an agent-scope atomic (it goes
-//    past the XCD's L2 to the device's coherence point);
-//  * ChaseXcc: only the workgroups on one XCD take part (the XCD of the first
-//    workgroup to arrive; the others exit at once).  That XCD's L2 is then the
-//    coherence point of every participant: loads bypass the CU's L1 (the sc0
-//    cache policy: coherent across the CUs that share an L2) and hit the L2,
-//    stores write through the L1 into it.  Far shorter round trips than the
-//    device-scope accesses, which is what the sweep chain is made of.
-// In both, a task's completion is published only after all its stores have
-// been acknowledged (s_waitcnt), and a consumer issues its loads only after it
-// has read the published progress.
+// Memory access of the bulge chase.  Band elements, progress counters and
+// mailbox slots move between workgroups that may sit on any XCD, whose L2
+// caches are not coherent with each other: every access is an agent-scope
+// atomic (it goes past the XCD's L2 to the device's coherence point; no L2
+// write-back or invalidate is needed).  A task's completion is published only
+// after all its stores have been acknowledged (s_waitcnt), and a consumer
+// issues its loads only after it has read the published progress.
 struct ChaseArgs {
   double *AB;
   int *prog;
   unsigned long long *mail;
   int *flag;
-  long n_ab, n_prog, n_mail;  // element counts
 };
 
 struct ChaseAgent {
@@ -1134,62 +1126,6 @@ struct ChaseAgent {
   __device__ void st_abort() const {
     __hip_atomic_store(&flag[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-};
-
-// cache policy bits of the buffer intrinsics: sc0 (coherent at the L2 shared
-// by the CUs of one XCD) and the compiler's volatile flag (no hoisting or
-// merging of a polled load)
-constexpr int kSc0 = 1;
-constexpr int kSc0Volatile = 1 | (1 << 31);
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-
-struct ChaseXcc {
-  int *flag;
-  __amdgpu_buffer_rsrc_t rab, rprog, rmail, rflag;
-  __device__ explicit ChaseXcc(const ChaseArgs &a)
-      : flag(a.flag),
-        rab(__builtin_amdgcn_make_buffer_rsrc(a.AB, 0, (int)(a.n_ab * 8), 0x00020000)),
-        rprog(__builtin_amdgcn_make_buffer_rsrc(a.prog, 0, (int)(a.n_prog * 4), 0x00020000)),
-        rmail(__builtin_amdgcn_make_buffer_rsrc(a.mail, 0, (int)(a.n_mail * 8), 0x00020000)),
-        rflag(__builtin_amdgcn_make_buffer_rsrc(a.flag, 0, 16, 0x00020000)) {}
-  // true when this workgroup takes part: its XCD is the one that the first
-  // workgroup to arrive claimed (flag[1], -1 before the launch)
-  __device__ bool init(bool lead) {
-    int mine = 0;
-    if (lead) {
-      const int xcc = (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15);
-      int expect = -1;
-      __hip_atomic_compare_exchange_strong(&flag[1], &expect, xcc, __ATOMIC_RELAXED,
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      mine = (expect == -1 || expect == xcc) ? 1 : 0;
-    }
-    return __shfl(mine, 0, 64) != 0;
-  }
-  __device__ double ld(long e) const {
-    return __builtin_bit_cast(double,
-                              __builtin_amdgcn_raw_buffer_load_b64(rab, (int)(e * 8), 0, kSc0));
-  }
-  __device__ void st(long e, double v) const {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rab, (int)(e * 8), 0, 0);
-  }
-  __device__ int ld_prog(int s) const {
-    return (int)__builtin_amdgcn_raw_buffer_load_b32(rprog, s * 4, 0, kSc0Volatile);
-  }
-  __device__ void st_prog(int s, int v) const {
-    __builtin_amdgcn_raw_buffer_store_b32((unsigned)v, rprog, s * 4, 0, 0);
-  }
-  __device__ unsigned long long ld_mail(long e) const {
-    return __builtin_bit_cast(unsigned long long,
-                              __builtin_amdgcn_raw_buffer_load_b64(rmail, (int)(e * 8), 0,
-                                                                   kSc0Volatile));
-  }
-  __device__ void st_mail(long e, unsigned long long v) const {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rmail, (int)(e * 8), 0, 0);
-  }
-  __device__ int ld_abort() const {
-    return (int)__builtin_amdgcn_raw_buffer_load_b32(rflag, 0, 0, kSc0Volatile);
-  }
-  __device__ void st_abort() const { __builtin_amdgcn_raw_buffer_store_b32(1u, rflag, 0, 0, 0); }
 };
 
 // Empty mailbox slot: a signalling NaN, which no arithmetic result can be.
@@ -1236,7 +1172,7 @@ __device__ inline bool mail_wait(const M &mem, long e, double &v) {
 
 // Band storage AB[c * LDB + d] = A[c + d][c] for d <= B (zero for d > B and
 // past the matrix); clears the sweep progress counters, the abort flag, the
-// sweep claim counter and the mailboxes, and marks the chase's XCD unclaimed.
+// sweep claim counter and the mailboxes.
 __global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restrict__ A, long lda,
                                                   double *__restrict__ AB, int *prog, int *flag,
                                                   unsigned long long *mail, long nmail) {
@@ -1248,7 +1184,7 @@ __global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restric
     }
     if (e < nmail) mail[e] = kMailEmpty;
     if (e < n) prog[e] = 0;
-    if (e < 4) flag[e] = (e == 1) ? -1 : 0;
+    if (e < 4) flag[e] = 0;
   }
 }
 
@@ -1268,6 +1204,10 @@ __global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restric
 template <class M>
 __global__ __launch_bounds__(64) void k_sbr_chase(int n, ChaseArgs args, int mt) {
   __shared__ double vs[B], ys[B], us[B];
+  // band columns of the task staged between the coalesced memory layout
+  // (lane = diagonal of one column) and the compute layout; row pitch 66 so
+  // that the compute-layout reads T[c][r - c] hit 32 different banks
+  __shared__ double T[B][2 * B + 2];
   __shared__ int claim;
   const int lane = threadIdx.x, c = lane & 31, h = lane >> 5;
   M mem(args);
@@ -1304,20 +1244,31 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, ChaseArgs args, int mt)
           if (16 * h + i >= 1) part += L[i] * L[i];
       }
       asm volatile("" : "+v"(part)::"memory");
+      // D (lower triangle) and Bn are the diagonals 0 .. 2B - 1 of the band
+      // columns p .. p + B - 1: one column per load instruction, lane =
+      // diagonal, so each instruction reads 512 contiguous bytes
+      double colv[B];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int r = 16 * h + i;
-        D[i] = (r < l && c < l) ? (r >= c ? mem.ld((long)(p + c) * LDB + (r - c))
-                                          : mem.ld((long)(p + r) * LDB + (c - r)))
-                                : 0.0;
-        Bn[i] = (r < l2 && c < l) ? mem.ld((long)(p + c) * LDB + B + r - c) : 0.0;
-      }
+      for (int q = 0; q < B; ++q) colv[q] = (q < l) ? mem.ld((long)(p + q) * LDB + lane) : 0.0;
+      // the mailbox slot this task needs, read speculatively with the band
+      const bool need_mail = (s > 0 && k + 2 < ntp);
+      unsigned long long mail0 = kMailEmpty;
+      if (need_mail && lane == 63) mail0 = mem.ld_mail((long)(s - 1) * mt + k + 2);
       const double xn2 = __shfl(part, 0, 64) + __shfl(part, 32, 64);
       const double alp = __shfl(L[0], 0, 64);
       double tau, beta, sc;
       reflector(xn2, alp, tau, beta, sc);
       if (k >= 2 && lane == 0)
         mem.st_mail((long)s * mt + k, (unsigned long long)__double_as_longlong(beta));
+#pragma unroll
+      for (int q = 0; q < B; ++q) T[q][lane] = colv[q];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int r = 16 * h + i;
+        D[i] = (r < l && c < l) ? (r >= c ? T[c][r - c] : T[r][c - r]) : 0.0;
+        Bn[i] = (r < l2 && c < l) ? T[c][B + r - c] : 0.0;
+      }
       if (c == 0) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -1342,9 +1293,13 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, ChaseArgs args, int mt)
         for (int i = 0; i < 16; ++i) L[i] = (16 * h + i == 0) ? beta : 0.0;
       }
       // the head of task k + 2's L of sweep s - 1 (last row, last column of Bn)
-      if (s > 0 && k + 2 < ntp) {
+      if (need_mail) {
         double bm = 0.0;
-        if (!mail_wait(mem, (long)(s - 1) * mt + k + 2, bm)) return;
+        const unsigned long long m0 = __shfl(mail0, 63, 64);
+        if (m0 != kMailEmpty)
+          bm = __longlong_as_double((long long)m0);
+        else if (!mail_wait(mem, (long)(s - 1) * mt + k + 2, bm))
+          return;
         if (lane == 63) Bn[15] = bm;
       }
       // y = tau D v as column sums (D is exactly symmetric in registers: both
@@ -1375,7 +1330,10 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, ChaseArgs args, int mt)
         Bn[i] -= tau * us[16 * h + i] * vc;
       }
       // stores: L (its head from task 2 on belongs to sweep s + 1), the lower
-      // half of D, and Bn when no later task of this sweep takes it
+      // half of D, and Bn when no later task of this sweep takes it.  Each
+      // block goes through LDS so that a store instruction writes two band
+      // columns' contiguous runs (lanes 0-31 one column, 32-63 the next).
+      const int sr = lane & 31, sh = lane >> 5;
       if (k == 0) {
         if (c == 0) {
 #pragma unroll
@@ -1385,23 +1343,35 @@ __global__ __launch_bounds__(64) void k_sbr_chase(int n, ChaseArgs args, int mt)
           }
         }
       } else {
+        __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int r = 16 * h + i;
-          if (r < l && !(k >= 2 && r == 0 && c == 0))
-            mem.st((long)(p - B + c) * LDB + B + r - c, L[i]);
+        for (int i = 0; i < 16; ++i) T[c][16 * h + i] = L[i];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < B / 2; ++j) {
+          const int q = 2 * j + sh;  // column p - B + q, rows p + sr
+          if (sr < l && !(k >= 2 && sr == 0 && q == 0))
+            mem.st((long)(p - B + q) * LDB + B + sr - q, T[q][sr]);
         }
       }
+      __syncthreads();
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int r = 16 * h + i;
-        if (r < l && c < l && r >= c) mem.st((long)(p + c) * LDB + (r - c), D[i]);
+      for (int i = 0; i < 16; ++i) T[c][16 * h + i] = D[i];
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < B / 2; ++j) {
+        const int q = 2 * j + sh;  // column p + q, rows p + sr (lower part)
+        if (sr < l && q < l && sr >= q) mem.st((long)(p + q) * LDB + (sr - q), T[q][sr]);
       }
       if (k + 1 == nt && l2 > 0) {
+        __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int r = 16 * h + i;
-          if (r < l2 && c < l) mem.st((long)(p + c) * LDB + B + r - c, Bn[i]);
+        for (int i = 0; i < 16; ++i) T[c][16 * h + i] = Bn[i];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < B / 2; ++j) {
+          const int q = 2 * j + sh;  // column p + q, rows p + B + sr
+          if (sr < l2 && q < l) mem.st((long)(p + q) * LDB + B + sr - q, T[q][sr]);
         }
       }
       // publish task k once every lane's stores have completed
@@ -1448,13 +1418,6 @@ SbrSizes sbr_sizes(int n) {
   z.cq = (size_t)n / B + 4;  // two arrival counters, then one path flag per panel
   z.cqR = 2 * (size_t)B * B;
   return z;
-}
-
-// MBD_SBR_CHASE_XCC=0 runs the bulge chase on every XCD with device-scope
-// accesses instead of on one XCD through its L2
-bool sbr_chase_xcc() {
-  const char *e = getenv("MBD_SBR_CHASE_XCC");
-  return !(e && e[0] == '0');
 }
 
 bool sbr_lookahead() {
@@ -1604,16 +1567,9 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
     // about n^2 / (2B) tasks, each loading two B x B blocks and storing one and
     // a half (3.5 B^2 doubles) and doing about 12 B^2 flops
     ProfScope ps(K_SBR_CHASE, s, 8.0 * 3.5 * B * 0.5 * (double)n * n, 12.0 * B * 0.5 * (double)n * n);
-    const ChaseArgs ca{w.AB, w.prog, w.mail, w.flag, (long)zz.AB, (long)zz.prog, (long)zz.mail};
-    if (sbr_chase_xcc()) {
-      // 8x the workgroups (every XCD gets a share); those outside the chosen
-      // XCD exit at once
-      hipLaunchKernelGGL(k_sbr_chase<ChaseXcc>, dim3(8 * std::min(n - 2, kSbrChaseWG)), dim3(64),
-                         0, s, n, ca, sbr_mail_tasks(n));
-    } else {
-      hipLaunchKernelGGL(k_sbr_chase<ChaseAgent>, dim3(std::min(n - 2, kSbrChaseWG)), dim3(64),
-                         0, s, n, ca, sbr_mail_tasks(n));
-    }
+    const ChaseArgs ca{w.AB, w.prog, w.mail, w.flag};
+    hipLaunchKernelGGL(k_sbr_chase<ChaseAgent>, dim3(std::min(n - 2, kSbrChaseWG)), dim3(64), 0,
+                       s, n, ca, sbr_mail_tasks(n));
     MBD_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_sbr_de, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w.AB, w.d, w.e);

