@@ -253,6 +253,29 @@ int mbd_batch_config(int index, int *nb, int *nt, int *per_cu);
 int mbd_batch_run(mbd_handle *h, int mode, double *energy, double *alpha_eff, double *c6_eff,
                   double *c6_mol, double *alpha_all, int *stat, mbd_batch_diag *d);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the batched form of mbd_batch_run on caller-owned DEVICE
+ * arrays, allocating nothing.  The persistent kernel's scratch (per-workgroup
+ * matrix slots, counters) lives in a fixed device array of the library,
+ * about 46 MB per GPU, held until the process ends; its address is taken
+ * once per device by mbd_create.  Limits: molecules of at most 32 atoms
+ * (max_atoms), at most 16384 molecules per call; larger requests return
+ * MBD_ERR_TOOBIG.  Every call on one GPU uses that GPU's scratch: calls that
+ * overlap on one GPU (other streams, other handles, other host threads) race.
+ * Launches on `stream` (a hipStream_t; NULL = the handle's stream) and does
+ * not synchronise.  Arrays (device memory): offsets[nmol + 1] (offsets[0] =
+ * 0, natoms = offsets[nmol]), xyz[3 natoms] bohr, hirshfeld, alpha0, c6_0, r0
+ * [natoms]; outputs energy[nmol], alpha_all[21 natoms] (alpha_SCS at every
+ * grid point, [k natoms + i]), c6_eff[natoms], c6_mol[nmol], stat[4 nmol] as
+ * in mbd_batch_run.  Cutoffs in Angstrom. */
+int mbd_batch_run_noalloc(mbd_handle *h, void *stream, int mode, int nmol, int natoms,
+                          int max_atoms, const int *offsets, const double *xyz,
+                          const double *hirshfeld, const double *alpha0, const double *c6_0,
+                          const double *r0, double beta, double rc_scs, double rc_cfdm,
+                          double *energy, double *alpha_all, double *c6_eff, double *c6_mol,
+                          int *stat);
+
 /* Copies n_bytes (a multiple of 16) between two fresh device buffers with a
  * plain streaming kernel (k_copy_probe) and returns its duration in *ms:
  * the known-size reference that hardware byte counters are scaled by. */

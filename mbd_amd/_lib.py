@@ -101,6 +101,10 @@ def lib():
     L.mbd_batch_run.argtypes = [h, ctypes.c_int, _D, _D, _D, _D, _D, _I,
                                 ctypes.POINTER(MbdBatchDiag)]
     L.mbd_debug_copy.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.POINTER(ctypes.c_double)]
+    vp = ctypes.c_void_p
+    L.mbd_batch_run_noalloc.argtypes = ([h, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int] + [vp] * 6 + [ctypes.c_double] * 3
+                                        + [vp] * 5)
     for name in declared_symbols():
         getattr(L, name)  # every declared entry point must resolve
     _lib = L

@@ -29,7 +29,7 @@ struct BatchArgs {
   int nmol, ntask, nmax, S, natoms;
   int mode;  // 0: FP64 Cholesky; 1: FP32 Cholesky + FP64 refinement (FP64 fallback)
   const int *off;    // nmol + 1 atom offsets
-  const int *order;  // molecules in processing order (largest first)
+  const int *order;  // molecules in processing order (largest first); null: as given
   const double *xyz, *hirsh, *a0, *c60, *r0;  // per atom (bohr)
   const double *omega, *weight;              // 21-point grid
   double beta, rc_scs, rc_cfdm;              // cutoffs in bohr

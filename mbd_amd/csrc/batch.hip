@@ -567,7 +567,7 @@ __global__ __launch_bounds__(NT) void k_batch(BatchArgs a) {
     __syncthreads();
     const int t = ibuf[0];
     if (t >= a.ntask) break;
-    const int mol = a.order[t / kNumFreq], f = t % kNumFreq;
+    const int mol = a.order ? a.order[t / kNumFreq] : t / kNumFreq, f = t % kNumFreq;
     const int o = a.off[mol], N = a.off[mol + 1] - o, n = 3 * N;
     const int ld = n;
 

@@ -145,6 +145,25 @@ double norm3(const double *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2
 
 }  // namespace
 
+// Scratch of mbd_batch_run_noalloc: the persistent kernel's per-workgroup
+// matrix slots, the task and per-molecule counters, the residual maxima and
+// the phase timers, in one fixed device array (each GPU the library runs on
+// holds its own copy until the process ends).  Host code uses only the
+// device address that hipGetSymbolAddress returns for the current device.
+namespace {
+constexpr int kNaMaxAtoms = 32;     // atoms per molecule
+constexpr int kNaMaxMol = 16384;    // molecules per call
+constexpr int kNaSlots = 256;       // workgroups (one matrix slot each)
+constexpr size_t kNaSlotBytes = 180224;  // batch_slot_bytes(3 * kNaMaxAtoms)
+constexpr size_t kNaCountOff = (size_t)kNaSlots * kNaSlotBytes;
+constexpr size_t kNaResidOff = kNaCountOff + 4 * (size_t)(kNaMaxMol + 64);
+constexpr size_t kNaPhaseOff = kNaResidOff + 8 * (size_t)kNaMaxMol;
+constexpr size_t kNaBytes = kNaPhaseOff + 8 * 16;
+constexpr int kNaMaxDevices = 64;
+void *na_ptr[kNaMaxDevices] = {nullptr};
+}  // namespace
+static __device__ __align__(256) char scratch[kNaBytes];
+
 struct mbd_handle {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -736,6 +755,12 @@ int mbd_create(int device, mbd_handle **out) {
     mbd_casimir_grid(w, g);
     MBD_HIP(hipMemcpy(h->omega.p, w, sizeof w, hipMemcpyHostToDevice));
     MBD_HIP(hipMemcpy(h->weight.p, g, sizeof g, hipMemcpyHostToDevice));
+    // device address of the no-allocation scratch on this device (once)
+    if (device >= 0 && device < kNaMaxDevices && !na_ptr[device]) {
+      void *p = nullptr;
+      MBD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      na_ptr[device] = p;
+    }
   } catch (const HipError &e) {
     int rc = fail(h, e);
     delete h;
@@ -1844,6 +1869,85 @@ int mbd_batch_run(mbd_handle *h, int mode, double *energy, double *alpha_eff, do
       MBD_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device));
       for (int k = 0; k < kBatchPhases; ++k) d->phase_ms[k] = khz > 0 ? (double)ph[k] / khz : 0.0;
     }
+  } catch (const HipError &e) {
+    return fail(h, e);
+  }
+  return 0;
+}
+
+int mbd_batch_run_noalloc(mbd_handle *h, void *stream, int mode, int nmol, int natoms,
+                          int max_atoms, const int *offsets, const double *xyz,
+                          const double *hirshfeld, const double *alpha0, const double *c6_0,
+                          const double *r0, double beta, double rc_scs, double rc_cfdm,
+                          double *energy, double *alpha_all, double *c6_eff, double *c6_mol,
+                          int *stat) {
+  if (!h) return -1;
+  if (mode != MBD_MODE_FP64 && mode != MBD_MODE_MIXED) return -3;
+  if (nmol <= 0) return -4;
+  if (natoms < nmol) return -5;
+  if (max_atoms <= 0) return -6;
+  if (!offsets || !xyz || !hirshfeld || !alpha0 || !c6_0 || !r0) return -7;
+  if (!energy || !alpha_all || !c6_eff || !c6_mol || !stat) return -15;
+  if (max_atoms > kNaMaxAtoms || nmol > kNaMaxMol) {
+    char buf[160];
+    snprintf(buf, sizeof buf,
+             "mbd_batch_run_noalloc holds molecules of up to %d atoms and %d molecules per call "
+             "(asked: %d atoms, %d molecules)",
+             kNaMaxAtoms, kNaMaxMol, max_atoms, nmol);
+    h->err = buf;
+    return MBD_ERR_TOOBIG;
+  }
+  try {
+    MBD_HIP(hipSetDevice(h->device));
+    char *sc = h->device >= 0 && h->device < kNaMaxDevices ? (char *)na_ptr[h->device] : nullptr;
+    if (!sc) {
+      h->err = "mbd_batch_run_noalloc: no scratch address for this device";
+      return MBD_ERR_HIP;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    const int nmax = 3 * max_atoms;
+    const int cfg = batch_default_config(nmax);
+    const BatchConfig &c = kBatchConfigs[cfg];
+    if (batch_slot_bytes(nmax) > kNaSlotBytes) return MBD_ERR_TOOBIG;
+    int ncu = 0;
+    MBD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+    const int grid = std::max(1, std::min(ncu, kNaSlots));
+    BatchArgs a;
+    a.nmol = nmol;
+    a.ntask = nmol * kNumFreq;
+    a.nmax = nmax;
+    a.S = batch_lds_stride(nmax);
+    a.natoms = natoms;
+    a.mode = mode == MBD_MODE_FP64 ? 0 : 1;
+    a.off = offsets;
+    a.order = nullptr;
+    a.xyz = xyz;
+    a.hirsh = hirshfeld;
+    a.a0 = alpha0;
+    a.c60 = c6_0;
+    a.r0 = r0;
+    a.omega = h->omega.p;
+    a.weight = h->weight.p;
+    a.beta = beta;
+    a.rc_scs = rc_scs / kBohr;
+    a.rc_cfdm = rc_cfdm / kBohr;
+    a.work = sc;
+    a.slot_bytes = (long)kNaSlotBytes;
+    a.alpha = alpha_all;
+    a.c6 = c6_eff;
+    a.energy = energy;
+    a.c6_mol = c6_mol;
+    a.stat = stat;
+    a.resid = (double *)(sc + kNaResidOff);
+    a.counter = (int *)(sc + kNaCountOff);
+    a.phase = (unsigned long long *)(sc + kNaPhaseOff);
+    // counters, statistics and residual maxima start at zero (fill kernels)
+    batch_zero(s, a.counter, 1 + nmol);
+    batch_zero(s, stat, 4 * nmol);
+    batch_zero(s, (int *)a.resid, 2 * nmol);
+    batch_zero(s, (int *)a.phase, 2 * kBatchPhases);
+    batch_prepare(c, nmax);
+    batch_launch(s, c, a, grid);
   } catch (const HipError &e) {
     return fail(h, e);
   }
