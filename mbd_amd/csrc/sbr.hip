@@ -18,11 +18,11 @@
 //   gemm_nt      A22 -= V Z^T + Z V^T on the lower tiles (MFMA SYR2K).
 // Stage 2, band -> tridiagonal by bulge chasing (k_sbr_chase): sweep s
 // annihilates column s below the subdiagonal with a length-B reflector and
-// chases the bulge down in tasks of B rows; one wavefront per sweep, the
-// sweeps pipelined across a persistent grid: task k of sweep s starts once
-// sweep s - 1 has completed task k + 2, which leaves the two tasks' blocks
-// disjoint.  The band (2B diagonals per column) stays in HBM/L2; a task holds
-// its three 32 x 32 blocks in registers.
+// chases the bulge down in tasks of B rows.  A workgroup runs three
+// consecutive sweeps, one wavefront each, in lock step over a window of band
+// columns in LDS, with a fourth wavefront loading the columns the group's
+// first sweep needs from the previous group; groups pipeline across a
+// persistent grid through progress counters and mailboxes in memory.
 #include <algorithm>
 #include <cstdlib>
 
@@ -1059,27 +1059,6 @@ __global__ __launch_bounds__(256) void k_sbr_zero(int ncnt, int *cnt, int nm, in
 // ---- stage 2 --------------------------------------------------------------
 
 namespace {
-// Sum over the 32 lanes of a half-wave of q[0..15]; afterwards lane l holds
-// value ((l & 31) >> 1).
-__device__ inline double half_sum16(double (&q)[16], int lane) {
-#pragma unroll
-  for (int st = 0; st < 4; ++st) {
-    const int h = 8 >> st, mask = 16 >> st;
-    const bool up = (lane & mask) != 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k < h) {
-        double lo = q[k], hi = q[k + h];
-        asm volatile("" : "+v"(lo), "+v"(hi));
-        const double send = up ? lo : hi;
-        const double keep = up ? hi : lo;
-        q[k] = keep + __shfl_xor(send, mask, 64);
-      }
-    }
-  }
-  return q[0] + __shfl_xor(q[0], 1, 64);
-}
-
 // Memory access of the bulge chase.  Band elements, progress counters and
 // mailbox slots move between workgroups that may sit on any XCD, whose L2
 // caches are not coherent with each other: every access is an agent-scope
@@ -1188,201 +1167,498 @@ __global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restric
   }
 }
 
-// Bulge chasing, one wavefront per sweep; a workgroup claims sweeps in
-// increasing order from a counter (flag[2]), so every sweep it waits for has
-// been claimed by a running workgroup.
-// Task k of sweep s works on rows I = [p, p + B), p = s + 1 + k B: the
-// reflector from the first column of L = A[I, previous block] (k = 0: column
-// s), applied from the left to L, from both sides to D = A[I, I] and from the
-// right to Bn = A[next B rows, I], which stays in registers as the next
-// task's L.  Lane (h, c) = (lane >> 5, lane & 31) holds column c, rows
-// 16 h .. 16 h + 15 of each block.
-// Ordering with sweep s - 1: task k starts once s - 1 has completed task
-// k + 1; the one element it shares with task k + 2 of s - 1 (that task's
-// beta, the head of its L) arrives through a mailbox as soon as s - 1 has
-// formed the reflector, and task k + 2 leaves that element to sweep s.
+// Bulge chasing.  Task k of sweep s works on rows I = [p, p + B), p = s + 1 +
+// k B: the reflector from the first column of L = A[I, previous block]
+// (k = 0: column s), applied from the left to L, from both sides to
+// D = A[I, I] and from the right to Bn = A[next B rows, I], which stays in
+// registers as the next task's L.  Lane (h, c) = (lane >> 5, lane & 31) holds
+// column c, rows 16 h .. 16 h + 15 of each block.
+//
+// A workgroup runs a group of CS consecutive sweeps s0 .. s0 + CS - 1, one
+// compute wave per sweep, in lock step: at step t compute wave j runs task
+// t - 2 j, whose blocks are disjoint from those of wave j - 1's task
+// t - 2 j + 2 except for one element, the head of that task's L, which wave
+// j - 1 writes before the step's middle barrier and wave j reads after it.
+// The waves share a ring buffer of band columns in LDS (the window: WC
+// columns of 2B diagonals; the group's columns in flight span at most
+// 2 B CS - CS + 2).  A band column crosses between workgroups once per CS
+// sweeps:
+//  * a loading wave brings the columns of the group's first sweep into the
+//    window, one task ahead: it loads task k + 1's columns once the previous
+//    group's last sweep has completed task k + 2 (progress counter), and the
+//    head of that sweep's task k + 3 from a mailbox.  Memory holds that
+//    sweep's values only inside its coverage (diagonals 0 .. 2B - 2 - q of
+//    the task's column q); the loading wave zeroes the rest, which the band
+//    holds as zeros after that sweep but memory may hold as an older group's
+//    value.  It has no arithmetic to do, so the loads stay in flight across
+//    a whole step without taking registers from the compute waves;
+//  * the group's last sweep stores what its task wrote and publishes task
+//    k - 1 once every memory operation it issued before task k's stores has
+//    completed: a wave's memory operations complete in order, so
+//    s_waitcnt vmcnt(#task k stores) is that condition, and the stores of
+//    one task stay in flight while the next step computes.
+// Groups are claimed in increasing order from a counter (flag[2]), so every
+// group a workgroup waits for has been claimed by a running workgroup.
+constexpr int CW = kSbrChaseWaves;
+constexpr int CS = CW - 1;
+constexpr int WC = 2 * B * CW;
+constexpr int PW = 2 * B + 2;  // window column pitch: T[c][r - c] reads hit distinct banks
+static_assert((WC & (WC - 1)) == 0, "window size must be a power of two");
+static_assert(CS >= 2 && 2 * B * CS - CS + 2 + B <= WC,
+              "the window must hold the columns in flight and the next task's");
+static_assert(CS <= B, "a group's sweep columns must lie in its first task's block");
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Workgroup barrier that orders LDS only.  __syncthreads() also waits for the
+// wave's outstanding memory loads and stores (s_waitcnt vmcnt(0)), which
+// would put the band loads and the last sweep's band stores on every step's
+// critical path.
+__device__ inline void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The window: band column col, diagonal d.
+__device__ inline double &wcol(double *win, int col, int d) { return win[(col & (WC - 1)) * PW + d]; }
+
+// The value of lane l ^ 32 (v_permlane32_swap: a VALU exchange of the two
+// half-waves, no LDS round trip).
+__device__ inline double xor32(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+  const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  const bool up = (threadIdx.x & 32) != 0;
+  const unsigned rlo = up ? a[0] : a[1], rhi = up ? b[0] : b[1];
+  return __longlong_as_double((long long)(((unsigned long long)rhi << 32) | rlo));
+}
+
+// The value of lane src, wave-uniform.
+__device__ inline double lane_value(double v, int src) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// 16-byte accesses to the band with the same cache policy as the agent-scope
+// atomics above (sc1: past the XCD's L2 to the device's coherence point).
+// Offsets outside the buffer read as zero and write nothing.
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+constexpr int kSc1 = 16;
+
+__device__ inline __amdgpu_buffer_rsrc_t band_rsrc(double *AB, int n) {
+  return __builtin_amdgcn_make_buffer_rsrc(AB, (short)0, n * LDB * 8, 0x00020000);
+}
+__device__ inline void ld2(__amdgpu_buffer_rsrc_t r, int e, double &a, double &b) {
+  const v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, e * 8, 0, kSc1);
+  a = __longlong_as_double((long long)(((unsigned long long)v[1] << 32) | v[0]));
+  b = __longlong_as_double((long long)(((unsigned long long)v[3] << 32) | v[2]));
+}
+__device__ inline void st2(__amdgpu_buffer_rsrc_t r, int e, double a, double b) {
+  const unsigned long long x = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long y = (unsigned long long)__double_as_longlong(b);
+  const v4u v = {(unsigned)x, (unsigned)(x >> 32), (unsigned)y, (unsigned)(y >> 32)};
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, e * 8, 0, kSc1);
+}
+
+// Loads for task k of sweep s: the task's columns, two per load instruction
+// (lanes 0-31 column 2 jj, 32-63 column 2 jj + 1, two diagonals per lane;
+// columns past n - 1 read as zero), at the last task column p + B when it is
+// column n - 1, and an early read of the mailbox slot with the previous
+// sweep's head for the task.
 template <class M>
-__global__ __launch_bounds__(64) void k_sbr_chase(int n, ChaseArgs args, int mt) {
-  __shared__ double vs[B], ys[B], us[B];
-  // band columns of the task staged between the coalesced memory layout
-  // (lane = diagonal of one column) and the compute layout; row pitch 66 so
-  // that the compute-layout reads T[c][r - c] hit 32 different banks
-  __shared__ double T[B][2 * B + 2];
-  __shared__ int claim;
-  const int lane = threadIdx.x, c = lane & 31, h = lane >> 5;
-  M mem(args);
-  if (!mem.init(lane == 0)) return;
-  double L[16], D[16], Bn[16];
-  for (;;) {
-    if (lane == 0)
-      claim = __hip_atomic_fetch_add(&mem.flag[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int s = claim;
-    __syncthreads();
-    if (s > n - 3) return;
-    const int nt = (n - 3 - s) / B + 1;
-    const int ntp = (s > 0) ? (n - 2 - s) / B + 1 : 0;
-    for (int k = 0; k < nt; ++k) {
-      if (s > 0 && !chase_wait(mem, s - 1, min(k + 2, ntp))) return;
-      asm volatile("" ::: "memory");
-      const int p = s + 1 + B * k;
-      const int l = min(B, n - p);
-      const int l2 = max(0, min(B, n - p - B));
-      if (k == 0) {
+__device__ inline void chase_load(const M &mem, __amdgpu_buffer_rsrc_t rab, int n, int mt, int s,
+                                  int k, int nt, int ntp, int lane, double (&colv)[B],
+                                  double &colx, unsigned long long &mail0) {
+  const int p = s + 1 + B * k;
+  const int lc = (k + 1 == nt) ? n - p : min(B, n - p);
+  const int sr = lane & 31, sh = lane >> 5;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int r = 16 * h + i;
-          L[i] = (c == 0 && r < l) ? mem.ld((long)s * LDB + 1 + r) : 0.0;
+  for (int jj = 0; jj < B / 2; ++jj) ld2(rab, (p + 2 * jj + sh) * LDB + 2 * sr, colv[2 * jj], colv[2 * jj + 1]);
+  colx = (lc > B) ? mem.ld((long)(p + B) * LDB + lane) : 0.0;
+  mail0 = (s > 0 && k + 2 < ntp && lane == 0) ? mem.ld_mail((long)(s - 1) * mt + k + 2) : kMailEmpty;
+}
+
+// Task k's loaded columns into the window, restricted to the previous
+// sweep's coverage, with its head from the mailbox.  False if the mailbox
+// wait gave up.
+template <class M>
+__device__ inline bool chase_put(const M &mem, int n, int mt, int s, int k, int nt, int ntp, int lane,
+                                 double *win, const double (&colv)[B], double colx,
+                                 unsigned long long mail0) {
+  const int p = s + 1 + B * k;
+  const bool need_mail = s > 0 && k + 2 < ntp;
+  double bm = 0.0;
+  if (need_mail) {
+    const unsigned long long m0 = __shfl(mail0, 0, 64);
+    if (m0 != kMailEmpty)
+      bm = __longlong_as_double((long long)m0);
+    else if (!mail_wait(mem, (long)(s - 1) * mt + k + 2, bm))
+      return false;
+  }
+  const int lc = (k + 1 == nt) ? n - p : min(B, n - p);
+  const int sr = lane & 31, sh = lane >> 5;
+#pragma unroll
+  for (int jj = 0; jj < B / 2; ++jj) {
+    const int q = 2 * jj + sh;
+    double v[2] = {colv[2 * jj], colv[2 * jj + 1]};
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int d = 2 * sr + e;
+      if (s > 0) {
+        if (q == B - 1) {
+          if (d > B) v[e] = 0.0;
+          else if (d == B && need_mail) v[e] = bm;  // the previous sweep's next head
+        } else if (d >= max(1, 2 * B - 1 - q)) {
+          v[e] = 0.0;
         }
       }
-      // |x|^2 below the head of column 0 of L; consumed before the D / Bn
-      // loads are issued, so the reflector never waits for them
-      double part = 0.0;
-      if (c == 0) {
+    }
+    if (q < lc) {
+      double *w = &wcol(win, p + q, 2 * sr);
+      w[0] = v[0];
+      w[1] = v[1];
+    }
+  }
+  if (lc > B) wcol(win, p + B, lane) = (s > 0 && lane >= B - 1) ? 0.0 : colx;
+  return true;
+}
+
+__device__ inline int chase_steps(int n, int s0, int last) {
+  return max((n - 3 - s0) / B + 1, (n - 3 - s0 - last) / B + 1 + 2 * last);
+}
+
+// The loading wave of the group s0 .. s0 + last: before the first step,
+// sweep column s0 and task 0's columns; in step t, the loads of task t + 1's
+// columns (phase A) and their writes into the window (phase B), where the
+// first sweep's wave reads them in step t + 1.  Returns false when the chase
+// is abandoned (a bounded wait gave up).
+template <class M>
+__device__ __forceinline__ bool chase_loader(const M &mem, __amdgpu_buffer_rsrc_t rab, int n, int mt,
+                                             int s0, int last, double *win, int *sh_abort) {
+  const int lane = threadIdx.x & 63;
+  const int s = s0;
+  const int nt = (n - 3 - s) / B + 1;
+  const int ntp = s0 > 0 ? (n - 2 - s0) / B + 1 : 0;  // tasks of sweep s0 - 1
+  const int steps = chase_steps(n, s0, last);
+  double colv[B], colx;
+  unsigned long long mail0;
+  bool ok = s == 0 || chase_wait(mem, s - 1, min(2, ntp));
+  if (ok) {
+    const double v = mem.ld((long)s * LDB + lane);
+    chase_load(mem, rab, n, mt, s, 0, nt, ntp, lane, colv, colx, mail0);
+    wcol(win, s, lane) = (lane <= B) ? v : 0.0;
+    ok = chase_put(mem, n, mt, s, 0, nt, ntp, lane, win, colv, colx, mail0);
+  }
+  if (!ok && lane == 0) *sh_abort = 1;
+  lds_barrier();
+  if (*sh_abort) return false;
+  for (int t = 0; t < steps; ++t) {
+    const bool next = t + 1 < nt;
+    if (next) {
+      ok = s == 0 || chase_wait(mem, s - 1, min(t + 3, ntp));
+      if (ok)
+        chase_load(mem, rab, n, mt, s, t + 1, nt, ntp, lane, colv, colx, mail0);
+      else if (lane == 0)
+        *sh_abort = 1;
+    }
+    lds_barrier();
+    if (*sh_abort) return false;
+    if (next && !chase_put(mem, n, mt, s, t + 1, nt, ntp, lane, win, colv, colx, mail0) && lane == 0)
+      *sh_abort = 1;
+    lds_barrier();
+    if (*sh_abort) return false;
+  }
+  return true;
+}
+
+// Window addresses of the elements a lane (h, c) holds in a task at p
+// (r = 16 h + i): L = A[p + r][p - B + c], D below the diagonal A[p + r][p + c]
+// (r >= c), D above it A[p + c][p + r] (r < c, read through the symmetry) and
+// Bn[c][r] = A[p + B + c][p + r].  LIN: the task's columns p - B .. p + B - 1
+// lie in the window without wrapping, so every address is one of three
+// per-lane bases plus a constant.
+template <bool LIN>
+struct TaskCols {
+  double *win;
+  int p, c, h;
+  double *bL, *bD, *bU;
+  __device__ TaskCols(double *w, int p_, int lane) : win(w), p(p_), c(lane & 31), h(lane >> 5) {
+    if (LIN) {
+      double *cb = win + ((p - B) & (WC - 1)) * PW;
+      bL = cb + c * (PW - 1) + B + 16 * h;
+      bD = cb + B * PW + c * (PW - 1) + 16 * h;
+      bU = cb + (B + 16 * h) * PW + c - 16 * h;
+    }
+  }
+  __device__ double *col(int x) const { return win + ((p + x) & (WC - 1)) * PW; }
+  __device__ double *L(int i) const { return LIN ? bL + i : col(c - B) + B + 16 * h + i - c; }
+  __device__ double *Dlo(int i) const { return LIN ? bD + i : col(c) + 16 * h + i - c; }
+  __device__ double *Dup(int i) const {
+    return LIN ? bU + i * (PW - 1) : col(16 * h + i) + c - 16 * h - i;
+  }
+  __device__ double *Bn(int i) const {
+    return LIN ? bU + B + i * (PW - 1) : col(16 * h + i) + B + c - 16 * h - i;
+  }
+};
+
+// Phase A of task k of sweep s: L in column form (k = 0: column s) and the
+// reflector of its first column, v into vs.  FULL: an interior task (k >= 2,
+// full blocks), which needs no masks.
+template <bool FULL, bool LIN>
+__device__ __forceinline__ void task_a(const TaskCols<LIN> &tc, double *win, int s, int k, int l,
+                                       int lane, double (&L)[16], double &tau, double &beta,
+                                       double &sc, double *vs) {
+  const int c = lane & 31, h = lane >> 5;
+  if (!FULL && k == 0) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
-          if (16 * h + i >= 1) part += L[i] * L[i];
+    for (int i = 0; i < 16; ++i) {
+      const int r = 16 * h + i;
+      const double v = wcol(win, s, 1 + r);
+      L[i] = (c == 0 && r < l) ? v : 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const double v = *tc.L(i);
+      L[i] = (FULL || 16 * h + i < l) ? v : 0.0;
+    }
+  }
+  double part = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if (16 * h + i >= 1) part += L[i] * L[i];
+  const double xn2 = lane_value(part, 0) + lane_value(part, 32);
+  const double alp = lane_value(L[0], 0);
+  reflector(xn2, alp, tau, beta, sc);
+  if (c == 0) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int r = 16 * h + i;
+      vs[r] = (r == 0) ? 1.0 : sc * L[i];
+    }
+  }
+}
+
+// Phase B of task k: H L, H D H and Bn H into the window.  Lane (h, c) holds
+// rows 16 h .. 16 h + 15 of column c of L and D and columns 16 h .. 16 h + 15
+// of row c of Bn.  Window reads are unconditional (every address is a valid
+// slot) with the value masked; a write a lane must not make goes to its own
+// dummy slot (mine), so no branch splits the wave.
+template <bool FULL, bool LIN>
+__device__ __forceinline__ void task_b(const TaskCols<LIN> &tc, double *win, int s, int k, int l,
+                                       int l2, int lane, double (&L)[16], double tau, double beta,
+                                       const double *vs, double *ys, double *mine) {
+  const int c = lane & 31, h = lane >> 5;
+  double D[16], Bn[16], vr[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int r = 16 * h + i;
+    // D is exactly symmetric: both triangles come from the same elements
+    const double dv = *((r >= c) ? tc.Dlo(i) : tc.Dup(i));
+    const double bv = *tc.Bn(i);  // Bn[c][r]
+    D[i] = (FULL || (r < l && c < l)) ? dv : 0.0;
+    Bn[i] = (FULL || (c < l2 && r < l)) ? bv : 0.0;
+    vr[i] = vs[r];
+  }
+  const double vc = vs[c];
+  double wp = 0.0, yp = 0.0, up = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    wp += vr[i] * L[i];
+    yp += D[i] * vr[i];
+    up += Bn[i] * vr[i];
+  }
+  const double wc = wp + xor32(wp);    // (v^T L) of column c
+  const double ycol = yp + xor32(yp);  // (D v) of row c
+  const double u = up + xor32(up);     // (Bn v) of row c
+  // left: L -= tau v (v^T L); right: Bn -= tau (Bn v) v^T
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    L[i] -= tau * vr[i] * wc;
+    Bn[i] -= tau * u * vr[i];
+  }
+  if (c == 0) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) L[i] = (16 * h + i == 0) ? beta : 0.0;
+  }
+  if (h == 0) ys[c] = tau * ycol;
+  wave_sync();
+  double yr[16];
+  double kp = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    yr[i] = ys[16 * h + i];
+    kp += vr[i] * yr[i];
+  }
+  const double K = 0.5 * tau * (kp + xor32(kp));
+  const double yc = ys[c] - K * vc;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) D[i] -= vr[i] * yc + (yr[i] - K * vr[i]) * vc;
+  // the task's results into the window: L (its head from task 2 on was
+  // written in phase A), the lower half of D and Bn
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int r = 16 * h + i;
+    if (!FULL && k == 0) {
+      *((c == 0 && r < l) ? &wcol(win, s, 1 + r) : mine) = L[i];
+    } else {
+      *(((FULL || r < l) && !((FULL || k >= 2) && r == 0 && c == 0)) ? tc.L(i) : mine) = L[i];
+    }
+    *((r >= c && (FULL || (r < l && c < l))) ? tc.Dlo(i) : mine) = D[i];
+    *((FULL || (c < l2 && r < l)) ? tc.Bn(i) : mine) = Bn[i];
+  }
+}
+
+// Compute wave j of the group s0 .. s0 + last (sweep s0 + j); stores: it is
+// the group's last sweep.  Every task writes its Bn to the window, where the
+// next task reads it back as L in column form.  Returns false when the chase
+// is abandoned.
+template <class M>
+__device__ __forceinline__ bool chase_sweep(const M &mem, __amdgpu_buffer_rsrc_t rab, int n, int mt,
+                                            int s0, int last, int j, bool stores, double *win,
+                                            double *vs, double *ys, double *dmy, int *sh_abort) {
+  const int lane = threadIdx.x & 63;
+  const int s = s0 + j;
+  const int nt = j <= last ? (n - 3 - s) / B + 1 : 0;
+  const int steps = chase_steps(n, s0, last);
+  double *const mine = dmy + lane;
+  // Nothing of this wave is in flight here (the storing wave's last task
+  // waited for its stores).  Saying so keeps the compiler, which merges the
+  // loading wave's code paths into this one's, from waiting for phantom
+  // loads inside the step loop, where the storing wave's stores must stay in
+  // flight: s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15).
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  lds_barrier();  // the loading wave's first columns
+  if (*sh_abort) return false;
+  for (int t = 0; t < steps; ++t) {
+    const int k = t - 2 * j;
+    const bool act = k >= 0 && k < nt;
+    const int p = s + 1 + B * k;
+    const int l = min(B, n - p);
+    const int l2 = max(0, min(B, n - p - B));
+    double L[16];
+    double tau = 0.0, beta = 0.0, sc = 0.0;
+    // interior tasks (full blocks, the head rule in force) need no masks;
+    // most of them also lie in the window without wrapping
+    const bool full = k >= 2 && l == B && l2 == B;
+    const bool lin = ((p - B) & (WC - 1)) + 2 * B <= WC;
+    const TaskCols<true> tl(win, p, lane);
+    const TaskCols<false> tw(win, p, lane);
+    // ---- phase A: L and its reflector
+    if (act) {
+      if (full && lin)
+        task_a<true>(tl, win, s, k, l, lane, L, tau, beta, sc, vs);
+      else if (full)
+        task_a<true>(tw, win, s, k, l, lane, L, tau, beta, sc, vs);
+      else
+        task_a<false>(tw, win, s, k, l, lane, L, tau, beta, sc, vs);
+      if (k >= 2 && lane == 0) {
+        wcol(win, p - B, B) = beta;  // the head of L, read by the next wave after the barrier
+        if (stores) mem.st_mail((long)s * mt + k, (unsigned long long)__double_as_longlong(beta));
       }
-      asm volatile("" : "+v"(part)::"memory");
-      // D (lower triangle) and Bn are the diagonals 0 .. 2B - 1 of the band
-      // columns p .. p + B - 1: one column per load instruction, lane =
-      // diagonal, so each instruction reads 512 contiguous bytes
-      double colv[B];
+    }
+    lds_barrier();
+    if (*sh_abort) return false;
+    // ---- phase B: the two-sided update
+    if (act) {
+      if (full && lin)
+        task_b<true>(tl, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);
+      else if (full)
+        task_b<true>(tw, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);
+      else
+        task_b<false>(tw, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);
+      if (stores) {
+        // stores from the window, 16 bytes per lane: four band columns per
+        // instruction; every value is read before the first store is issued
+        wave_sync();
+        const int m2 = 2 * (lane & 15), q4 = lane >> 4;
+        const bool lastk = k + 1 == nt;
+        if (k == 0) {
+          for (int cc = s0; cc <= s; ++cc)
+            if (lane < (cc < s ? 2 : 1 + l)) mem.st((long)cc * LDB + lane, wcol(win, cc, lane));
+        } else {
+          // the previous block's columns p - B + qq at diagonals B - qq + r
+          double lv[B / 4][2];
 #pragma unroll
-      for (int q = 0; q < B; ++q) colv[q] = (q < l) ? mem.ld((long)(p + q) * LDB + lane) : 0.0;
-      // the mailbox slot this task needs, read speculatively with the band
-      const bool need_mail = (s > 0 && k + 2 < ntp);
-      unsigned long long mail0 = kMailEmpty;
-      if (need_mail && lane == 63) mail0 = mem.ld_mail((long)(s - 1) * mt + k + 2);
-      const double xn2 = __shfl(part, 0, 64) + __shfl(part, 32, 64);
-      const double alp = __shfl(L[0], 0, 64);
-      double tau, beta, sc;
-      reflector(xn2, alp, tau, beta, sc);
-      if (k >= 2 && lane == 0)
-        mem.st_mail((long)s * mt + k, (unsigned long long)__double_as_longlong(beta));
+          for (int jj = 0; jj < B / 4; ++jj) {
+            const int qq = 4 * jj + q4;
+            const double *w = &wcol(win, p - B + qq, B - qq + m2);
+            lv[jj][0] = w[0];
+            lv[jj][1] = w[1];
+          }
 #pragma unroll
-      for (int q = 0; q < B; ++q) T[q][lane] = colv[q];
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int r = 16 * h + i;
-        D[i] = (r < l && c < l) ? (r >= c ? T[c][r - c] : T[r][c - r]) : 0.0;
-        Bn[i] = (r < l2 && c < l) ? T[c][B + r - c] : 0.0;
-      }
-      if (c == 0) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int r = 16 * h + i;
-          vs[r] = (r == 0) ? 1.0 : sc * L[i];
-        }
-      }
-      __syncthreads();
-      double vr[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) vr[i] = vs[16 * h + i];
-      const double vc = vs[c];
-      // left: L -= tau v (v^T L)
-      double wp = 0.0;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) wp += vr[i] * L[i];
-      const double wc = wp + __shfl_xor(wp, 32, 64);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) L[i] -= tau * vr[i] * wc;
-      if (c == 0) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) L[i] = (16 * h + i == 0) ? beta : 0.0;
-      }
-      // the head of task k + 2's L of sweep s - 1 (last row, last column of Bn)
-      if (need_mail) {
-        double bm = 0.0;
-        const unsigned long long m0 = __shfl(mail0, 63, 64);
-        if (m0 != kMailEmpty)
-          bm = __longlong_as_double((long long)m0);
-        else if (!mail_wait(mem, (long)(s - 1) * mt + k + 2, bm))
-          return;
-        if (lane == 63) Bn[15] = bm;
-      }
-      // y = tau D v as column sums (D is exactly symmetric in registers: both
-      // triangles come from the same stored elements), u = Bn v as row sums
-      double yp = 0.0;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) yp += D[i] * vr[i];
-      const double ycol = yp + __shfl_xor(yp, 32, 64);
-      double q[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) q[i] = Bn[i] * vc;
-      const double usum = half_sum16(q, lane);
-      if (h == 0) ys[c] = tau * ycol;
-      if ((c & 1) == 0) us[16 * h + (c >> 1)] = usum;
-      __syncthreads();
-      double yr[16];
-      double kp = 0.0;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        yr[i] = ys[16 * h + i];
-        kp += vr[i] * yr[i];
-      }
-      const double K = 0.5 * tau * (kp + __shfl_xor(kp, 32, 64));
-      const double yc = ys[c] - K * vc;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        D[i] -= vr[i] * yc + (yr[i] - K * vr[i]) * vc;
-        Bn[i] -= tau * us[16 * h + i] * vc;
-      }
-      // stores: L (its head from task 2 on belongs to sweep s + 1), the lower
-      // half of D, and Bn when no later task of this sweep takes it.  Each
-      // block goes through LDS so that a store instruction writes two band
-      // columns' contiguous runs (lanes 0-31 one column, 32-63 the next).
-      const int sr = lane & 31, sh = lane >> 5;
-      if (k == 0) {
-        if (c == 0) {
-#pragma unroll
-          for (int i = 0; i < 16; ++i) {
-            const int r = 16 * h + i;
-            if (r < l) mem.st((long)s * LDB + 1 + r, L[i]);
+          for (int jj = 0; jj < B / 4; ++jj) {
+            const int qq = 4 * jj + q4;
+            st2(rab, (p - B + qq) * LDB + B - qq + m2, lv[jj][0], lv[jj][1]);
           }
         }
-      } else {
-        __syncthreads();
+        // diagonals 0 .. B - 1 of the task's columns: the lower half of D
+        // (the rest is Bn, stored again as the next task's L)
+        double dv[B / 4][2];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) T[c][16 * h + i] = L[i];
-        __syncthreads();
+        for (int jj = 0; jj < B / 4; ++jj) {
+          const int qq = 4 * jj + q4;
+          const double *w = &wcol(win, p + qq, m2);
+          dv[jj][0] = w[0];
+          dv[jj][1] = w[1];
+        }
 #pragma unroll
-        for (int j = 0; j < B / 2; ++j) {
-          const int q = 2 * j + sh;  // column p - B + q, rows p + sr
-          if (sr < l && !(k >= 2 && sr == 0 && q == 0))
-            mem.st((long)(p - B + q) * LDB + B + sr - q, T[q][sr]);
+        for (int jj = 0; jj < B / 4; ++jj) {
+          const int qq = 4 * jj + q4;
+          if (qq < l) st2(rab, (p + qq) * LDB + m2, dv[jj][0], dv[jj][1]);
+        }
+        if (lastk) {
+          // Bn has at most one row at a sweep's last task
+          if (l2 > 0 && lane < l)
+            mem.st((long)(p + lane) * LDB + B - lane, wcol(win, p + lane, B - lane));
+          if (p + l == n - 1 && lane == 0) mem.st((long)(n - 1) * LDB, wcol(win, n - 1, 0));
+          asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+          if (lane == 0) mem.st_prog(s, k + 1);
+        } else if (k >= 1) {
+          // the 2 x B / 4 store instructions above are the only memory
+          // operations issued since task k - 1's stores and task k's mailbox
+          asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+          if (lane == 0) mem.st_prog(s, k);
         }
       }
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < 16; ++i) T[c][16 * h + i] = D[i];
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < B / 2; ++j) {
-        const int q = 2 * j + sh;  // column p + q, rows p + sr (lower part)
-        if (sr < l && q < l && sr >= q) mem.st((long)(p + q) * LDB + (sr - q), T[q][sr]);
-      }
-      if (k + 1 == nt && l2 > 0) {
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 16; ++i) T[c][16 * h + i] = Bn[i];
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < B / 2; ++j) {
-          const int q = 2 * j + sh;  // column p + q, rows p + B + sr
-          if (sr < l2 && q < l) mem.st((long)(p + q) * LDB + B + sr - q, T[q][sr]);
-        }
-      }
-      // publish task k once every lane's stores have completed
-      __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_s_waitcnt(0);
-      asm volatile("" ::: "memory");
-      if (lane == 0) mem.st_prog(s, k + 1);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) L[i] = Bn[i];
     }
+    lds_barrier();
+    if (*sh_abort) return false;
+  }
+  return true;
+}
+
+template <class M>
+__global__ __launch_bounds__(64 * CW) void k_sbr_chase(int n, ChaseArgs args, int mt) {
+  __shared__ double win[WC * PW];
+  __shared__ double vsh[CS][B], ysh[CS][B], dmy[CS][64];
+  __shared__ int sh_group, sh_abort;
+  const int wave = threadIdx.x >> 6;
+  M mem(args);
+  if (!mem.init((threadIdx.x & 63) == 0)) return;
+  const __amdgpu_buffer_rsrc_t rab = band_rsrc(args.AB, n);
+  for (;;) {
+    if (threadIdx.x == 0) {
+      sh_group = __hip_atomic_fetch_add(&mem.flag[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      sh_abort = 0;
+    }
+    lds_barrier();
+    const int s0 = sh_group * CS;
+    if (s0 > n - 3) return;
+    const int last = min(CS, n - 2 - s0) - 1;  // the group's last sweep stores
+    const int j = wave - 1;
+    const bool ok = wave == 0 ? chase_loader(mem, rab, n, mt, s0, last, win, &sh_abort)
+                              : chase_sweep(mem, rab, n, mt, s0, last, j, j == last, win, vsh[j],
+                                            ysh[j], dmy[j], &sh_abort);
+    if (!ok) return;
   }
 }
 
@@ -1568,8 +1844,9 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
     // a half (3.5 B^2 doubles) and doing about 12 B^2 flops
     ProfScope ps(K_SBR_CHASE, s, 8.0 * 3.5 * B * 0.5 * (double)n * n, 12.0 * B * 0.5 * (double)n * n);
     const ChaseArgs ca{w.AB, w.prog, w.mail, w.flag};
-    hipLaunchKernelGGL(k_sbr_chase<ChaseAgent>, dim3(std::min(n - 2, kSbrChaseWG)), dim3(64), 0,
-                       s, n, ca, sbr_mail_tasks(n));
+    hipLaunchKernelGGL(k_sbr_chase<ChaseAgent>,
+                       dim3(std::min(ceil_div(n - 2, kSbrChaseWaves - 1), kSbrChaseWG)),
+                       dim3(64 * kSbrChaseWaves), 0, s, n, ca, sbr_mail_tasks(n));
     MBD_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_sbr_de, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, w.AB, w.d, w.e);
