@@ -1382,38 +1382,23 @@ __device__ __forceinline__ bool chase_loader(const M &mem, __amdgpu_buffer_rsrc_
 // Window addresses of the elements a lane (h, c) holds in a task at p
 // (r = 16 h + i): L = A[p + r][p - B + c], D below the diagonal A[p + r][p + c]
 // (r >= c), D above it A[p + c][p + r] (r < c, read through the symmetry) and
-// Bn[c][r] = A[p + B + c][p + r].  LIN: the task's columns p - B .. p + B - 1
-// lie in the window without wrapping, so every address is one of three
-// per-lane bases plus a constant.
-template <bool LIN>
+// Bn[c][r] = A[p + B + c][p + r].
 struct TaskCols {
   double *win;
   int p, c, h;
-  double *bL, *bD, *bU;
-  __device__ TaskCols(double *w, int p_, int lane) : win(w), p(p_), c(lane & 31), h(lane >> 5) {
-    if (LIN) {
-      double *cb = win + ((p - B) & (WC - 1)) * PW;
-      bL = cb + c * (PW - 1) + B + 16 * h;
-      bD = cb + B * PW + c * (PW - 1) + 16 * h;
-      bU = cb + (B + 16 * h) * PW + c - 16 * h;
-    }
-  }
+  __device__ TaskCols(double *w, int p_, int lane) : win(w), p(p_), c(lane & 31), h(lane >> 5) {}
   __device__ double *col(int x) const { return win + ((p + x) & (WC - 1)) * PW; }
-  __device__ double *L(int i) const { return LIN ? bL + i : col(c - B) + B + 16 * h + i - c; }
-  __device__ double *Dlo(int i) const { return LIN ? bD + i : col(c) + 16 * h + i - c; }
-  __device__ double *Dup(int i) const {
-    return LIN ? bU + i * (PW - 1) : col(16 * h + i) + c - 16 * h - i;
-  }
-  __device__ double *Bn(int i) const {
-    return LIN ? bU + B + i * (PW - 1) : col(16 * h + i) + B + c - 16 * h - i;
-  }
+  __device__ double *L(int i) const { return col(c - B) + B + 16 * h + i - c; }
+  __device__ double *Dlo(int i) const { return col(c) + 16 * h + i - c; }
+  __device__ double *Dup(int i) const { return col(16 * h + i) + c - 16 * h - i; }
+  __device__ double *Bn(int i) const { return col(16 * h + i) + B + c - 16 * h - i; }
 };
 
 // Phase A of task k of sweep s: L in column form (k = 0: column s) and the
 // reflector of its first column, v into vs.  FULL: an interior task (k >= 2,
 // full blocks), which needs no masks.
-template <bool FULL, bool LIN>
-__device__ __forceinline__ void task_a(const TaskCols<LIN> &tc, double *win, int s, int k, int l,
+template <bool FULL>
+__device__ __forceinline__ void task_a(const TaskCols &tc, double *win, int s, int k, int l,
                                        int lane, double (&L)[16], double &tau, double &beta,
                                        double &sc, double *vs) {
   const int c = lane & 31, h = lane >> 5;
@@ -1452,8 +1437,8 @@ __device__ __forceinline__ void task_a(const TaskCols<LIN> &tc, double *win, int
 // of row c of Bn.  Window reads are unconditional (every address is a valid
 // slot) with the value masked; a write a lane must not make goes to its own
 // dummy slot (mine), so no branch splits the wave.
-template <bool FULL, bool LIN>
-__device__ __forceinline__ void task_b(const TaskCols<LIN> &tc, double *win, int s, int k, int l,
+template <bool FULL>
+__device__ __forceinline__ void task_b(const TaskCols &tc, double *win, int s, int k, int l,
                                        int l2, int lane, double (&L)[16], double tau, double beta,
                                        const double *vs, double *ys, double *mine) {
   const int c = lane & 31, h = lane >> 5;
@@ -1546,17 +1531,12 @@ __device__ __forceinline__ bool chase_sweep(const M &mem, __amdgpu_buffer_rsrc_t
     const int l2 = max(0, min(B, n - p - B));
     double L[16];
     double tau = 0.0, beta = 0.0, sc = 0.0;
-    // interior tasks (full blocks, the head rule in force) need no masks;
-    // most of them also lie in the window without wrapping
+    // interior tasks (full blocks, the head rule in force) need no masks
     const bool full = k >= 2 && l == B && l2 == B;
-    const bool lin = ((p - B) & (WC - 1)) + 2 * B <= WC;
-    const TaskCols<true> tl(win, p, lane);
-    const TaskCols<false> tw(win, p, lane);
+    const TaskCols tw(win, p, lane);
     // ---- phase A: L and its reflector
     if (act) {
-      if (full && lin)
-        task_a<true>(tl, win, s, k, l, lane, L, tau, beta, sc, vs);
-      else if (full)
+      if (full)
         task_a<true>(tw, win, s, k, l, lane, L, tau, beta, sc, vs);
       else
         task_a<false>(tw, win, s, k, l, lane, L, tau, beta, sc, vs);
@@ -1569,9 +1549,7 @@ __device__ __forceinline__ bool chase_sweep(const M &mem, __amdgpu_buffer_rsrc_t
     if (*sh_abort) return false;
     // ---- phase B: the two-sided update
     if (act) {
-      if (full && lin)
-        task_b<true>(tl, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);
-      else if (full)
+      if (full)
         task_b<true>(tw, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);
       else
         task_b<false>(tw, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);

@@ -3,8 +3,9 @@
 Each molecule's energy, alpha_eff, C6 and molecular C6 are checked against
 the oracle (FP64 form within 1e-8 Ha, mixed form within 1e-6 Ha and C6 within
 1e-5 relative), the 512-molecule batch against its own FP64 form, graph
-replays against each other, and the autotuner's choice against the default
-launch shape."""
+replays against each other, every launch shape against the default one (FP64)
+and against the FP64 form (mixed), and the autotuner's choice in the mixed
+form."""
 import numpy as np
 import pytest
 
@@ -107,6 +108,33 @@ def test_every_launch_shape_matches_default():
             print(f"shape {c} max rel dE {worst[c]:.3e} at molecule {int(rel.argmax())} "
                   f"(E={ref['energy'][rel.argmax()]:.6e}) t={got['diag']['t_ms']:.2f} ms", flush=True)
     assert max(worst.values()) <= 1e-10, worst
+
+
+def test_tuner_mixed_over_every_shape():
+    """Every launch shape in the mixed form, forced in turn, against the FP64
+    form within the mixed tolerance (1 uHa), then the autotuner in the mixed
+    form: it times every shape, keeps the fastest, and the run after it uses
+    that shape."""
+    mols, st = generators.chain_molecules(n_mol=64, seed=11)
+    with engine.MbdEngine(0) as eng:
+        eng.batch_set(mols, st)
+        ref = eng.batch_run("fp64")
+        for c in range(engine.batch_num_configs()):
+            print(f"mixed shape {c} {engine.batch_config(c)}", flush=True)
+            eng.batch_set_config(c)
+            got = eng.batch_run("mixed")
+            assert got["diag"]["config"] == c
+            de = float(np.abs(got["energy"] - ref["energy"]).max())
+            print(f"mixed shape {c} max |dE| {de:.3e} t={got['diag']['t_ms']:.2f} ms", flush=True)
+            assert de <= 1e-6
+            assert got["diag"]["fallbacks"] == 0
+        best, t = eng.batch_tune("mixed", reps=2)
+        print(f"tuned: {best} times {t}", flush=True)
+        assert np.all(t > 0)
+        assert best == int(np.argmin(t))
+        got = eng.batch_run("mixed")
+        assert got["diag"]["config"] == best
+        assert float(np.abs(got["energy"] - ref["energy"]).max()) <= 1e-6
 
 
 def test_batch_rejects_bad_input():
