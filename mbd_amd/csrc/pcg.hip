@@ -302,28 +302,36 @@ __global__ __launch_bounds__(256) void k_cl_setup(const T *__restrict__ A, int l
   for (int e = t; e < m * m; e += 256) out[e] = Y[e / m][e % m];
 }
 
-// X0 = M^-1 B per cluster (alpha0 == nullptr), or X0_i = alpha0[b N + i] I.
+// X0 = M^-1 B per cluster, the solution with only intra-cluster screening.
+// With a prediction alpha0 of alpha_SCS (alpha0[b N + i]), each atom's
+// diagonal 3 x 3 block is shifted by (alpha0_i - tr(X0_ii) / 3) I, so its
+// trace carries the predicted screening from outside the cluster.
 __global__ __launch_bounds__(256) void k_cl_x0(int N, const int *__restrict__ off,
                                                const int *__restrict__ atoms,
                                                const long *__restrict__ moff, long mtot,
                                                const double *__restrict__ Minv,
                                                const double *__restrict__ alpha0,
                                                double *__restrict__ X, int ldx, long sX) {
+  __shared__ double dg[3 * kClMax];
   const int c = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
   const int a0 = off[c], m = 3 * (off[c + 1] - a0);
-  if (t >= m) return;
   double *Xb = X + (long)b * sX;
-  const int row = 3 * atoms[a0 + t / 3] + t % 3;
-  if (alpha0) {
-    const double al = alpha0[(long)b * N + atoms[a0 + t / 3]];
-    for (int k = 0; k < 3; ++k) Xb[row + (long)k * ldx] = (t % 3 == k) ? al : 0.0;
-    return;
-  }
   const double *Mi = Minv + (long)b * mtot + moff[c];
   double x[3] = {0.0, 0.0, 0.0};
-  // B has a 1 in row 3i + k of column k: column k of B restricted to the
-  // cluster selects the block columns j with j % 3 == k
-  for (int j = 0; j < m; ++j) x[j % 3] += Mi[t + (long)j * m];
+  if (t < m) {
+    // B has a 1 in row 3i + k of column k: column k of B restricted to the
+    // cluster selects the block columns j with j % 3 == k
+    for (int j = 0; j < m; ++j) x[j % 3] += Mi[t + (long)j * m];
+    dg[t] = x[t % 3];
+  }
+  __syncthreads();
+  if (t >= m) return;
+  if (alpha0) {
+    const int i0 = t - t % 3;
+    const double tr3 = (dg[i0] + dg[i0 + 1] + dg[i0 + 2]) / 3.0;
+    x[t % 3] += alpha0[(long)b * N + atoms[a0 + t / 3]] - tr3;
+  }
+  const int row = 3 * atoms[a0 + t / 3] + t % 3;
   for (int k = 0; k < 3; ++k) Xb[row + (long)k * ldx] = x[k];
 }
 

@@ -116,7 +116,12 @@ def test_device_training_reduces_loss():
 @pytest.mark.skipif(not os.path.exists(learned.DEFAULT_WEIGHTS), reason="no trained weights")
 def test_trained_network_warm_start_ab():
     """Held-out periodic cell: the learned form with and without the network.
-    Both must match the oracle; the iteration counts are reported."""
+    Both must match the oracle.  With the atom-cluster preconditioner the
+    iteration is started from the cluster solve M^-1 B; the network's
+    prediction only corrects each atom's diagonal block to the predicted
+    alpha_SCS.  Measured: the same iteration count with and without it
+    (12 here, 14 on a 1024-atom cell, 15 at C3) -- the network must not cost
+    iterations, and it does not save any."""
     blob = learned.load()
     geo, st = generators.random_periodic(384, 38.0 * (384 / 4096) ** (1 / 3), seed=999)
     eng = engine.MbdEngine(0)
@@ -135,4 +140,4 @@ def test_trained_network_warm_start_ab():
     assert abs(on.e_mbd - want.e_mbd) <= 1e-6 and abs(off.e_mbd - want.e_mbd) <= 1e-6
     assert on.diag["fallback_used"] == 0
     assert rmse < 0.1
-    assert on.diag["refine_iters"] <= off.diag["refine_iters"] + 2
+    assert on.diag["refine_iters"] <= off.diag["refine_iters"]
