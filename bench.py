@@ -67,10 +67,10 @@ def roofline_entry(prof, cls, bound):
 
 
 def measured_traffic_ratio(kernel):
-    """HBM bytes per launch of `kernel` over its algorithmic bytes, from the
-    committed rocprofv3 --pmc summaries (profiles/*traffic*.md: FETCH_SIZE +
-    WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy), or
-    None when no summary covers the kernel."""
+    """HBM bytes per launch of profiler class `kernel` over its algorithmic
+    bytes, from the committed rocprofv3 --pmc summaries (profiles/*traffic*.md:
+    FETCH_SIZE + WRITE_SIZE, each scaled by what the counter reads for a 1 GiB
+    copy), or None when no summary covers the class."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
         for line in open(path):
@@ -82,20 +82,27 @@ def measured_traffic_ratio(kernel):
 
 def traffic_probe(args):
     """Workload of the hardware-counter runs: one 1 GiB copy with the plain
-    copy kernel (the known-size reference), then the eigen stage at
-    n = 12288 on a seeded random symmetric matrix through the library's
-    test hook (same kernels and sizes as a C3 solve; everything on the null
-    stream).  Meant to run under rocprofv3 --pmc; prints progress only."""
+    copy kernel (the known-size reference), then one learned-form solve of the
+    benchmark cell with every kernel launched eagerly (no graph replay), its
+    SCS stage, CFDM build and two-stage eigen reduction at n = 12288.  Meant to
+    run under rocprofv3 --pmc; prints progress and, per profiler class, the
+    launches and algorithmic bytes of the solve (stderr) for the summary."""
     os.environ["MBD_TRD_GRAPH"] = "0"  # eager launches: one counter record per dispatch
-    from mbd_amd import engine
+    from mbd_amd import engine, generators
     ms = engine.device_copy(1 << 30)
     log(f"traffic-probe: copy 1 GiB in {ms:.3f} ms")
-    n = 12288
-    rng = np.random.default_rng(12288)
-    A = rng.standard_normal((n, n))
-    A = np.tril(A) + np.tril(A, -1).T
-    w = engine.device_sym_eigvals(A)
-    log(f"traffic-probe: eigen stage n={n} done, trace {w.sum():.6e} vs {np.trace(A):.6e}")
+    geo, st = generators.CONFIGS[args.config]()
+    eng = engine.MbdEngine(0)
+    eng.set_geometry(geo, st)
+    eng.profile_enable(True)
+    r = eng.run("learned")
+    prof = eng.profile_read()
+    eng.close()
+    log(f"traffic-probe: {args.config} learned solve E={r.e_mbd:.10f} iters={r.diag['refine_iters']}")
+    for k, v in prof.items():
+        if v["launches"]:
+            log(f"traffic-probe: class {k} launches {v['launches']} "
+                f"bytes_per_launch {v['bytes'] / v['launches']:.6e}")
 
 
 def batched_line(steps, warmup):
@@ -279,10 +286,11 @@ def main():
     if ratio is not None:
         out["roofline"]["traffic"] = ratio * out["roofline"]["algorithmic_per_launch"]
         out["roofline"]["traffic_basis"] = (
-            f"counter ratio {ratio} (FETCH_SIZE+WRITE_SIZE over algorithmic bytes, measured at "
-            f"n=4096) times the algorithmic bytes per launch of this run")
+            f"counter ratio {ratio} (FETCH_SIZE+WRITE_SIZE over algorithmic bytes, measured on "
+            f"the same workload) times the algorithmic bytes per launch of this run")
     others = {}
-    for k in ("trd_symv", "pcg_symv", "chol_syrk_gemm", "sbr_symm", "trd_syr2k_gemm", "sbr_chase"):
+    for k in ("scs_build", "cfdm_build", "pcg_symv", "refine_residual", "trd_symv", "chol_syrk_gemm",
+              "sbr_symm", "trd_syr2k_gemm", "sbr_chase"):
         if k in prof_m and prof_m[k]["launches"] and k != dom:
             others[k] = roofline_entry(prof_m, k, {"chol_syrk_gemm": "fp32_tflops",
                                                    "sbr_symm": "fp64_tflops",
@@ -291,8 +299,8 @@ def main():
             if r is not None and prof_m[k]["bytes"] > 0:
                 others[k]["traffic"] = r * prof_m[k]["bytes"] / max(prof_m[k]["launches"], 1)
                 others[k]["traffic_basis"] = (
-                    f"counter ratio {r} (FETCH_SIZE+WRITE_SIZE over algorithmic bytes, measured at "
-                    f"n=4096) times the algorithmic bytes per launch of this run")
+                    f"counter ratio {r} (FETCH_SIZE+WRITE_SIZE over algorithmic bytes, measured on "
+                    f"the same workload) times the algorithmic bytes per launch of this run")
     out["roofline_other"] = others
     eng.close()
     if not args.no_batch:
