@@ -1175,10 +1175,12 @@ __global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restric
 // column c, rows 16 h .. 16 h + 15 of each block.
 //
 // A workgroup runs a group of CS consecutive sweeps s0 .. s0 + CS - 1, one
-// compute wave per sweep, in lock step: at step t compute wave j runs task
-// t - 2 j, whose blocks are disjoint from those of wave j - 1's task
-// t - 2 j + 2 except for one element, the head of that task's L, which wave
-// j - 1 writes before the step's middle barrier and wave j reads after it.
+// compute wave per sweep, in lock step with one barrier per step: at step t
+// compute wave j runs task t - 2 j, whose blocks are disjoint from those of
+// wave j - 1's task t - 2 j + 2 except for one element, the head of that
+// task's L.  A wave forms its next task's reflector at the end of a step (that
+// task's L is the Bn the wave has just written) and writes the head into the
+// window then, before the barrier after which wave j + 1 reads it.
 // The waves share a ring buffer of band columns in LDS (the window: WC
 // columns of 2B diagonals; the group's columns in flight span at most
 // 2 B CS - CS + 2).  A band column crosses between workgroups once per CS
@@ -1336,10 +1338,10 @@ __device__ inline int chase_steps(int n, int s0, int last) {
 }
 
 // The loading wave of the group s0 .. s0 + last: before the first step,
-// sweep column s0 and task 0's columns; in step t, the loads of task t + 1's
-// columns (phase A) and their writes into the window (phase B), where the
-// first sweep's wave reads them in step t + 1.  Returns false when the chase
-// is abandoned (a bounded wait gave up).
+// sweep column s0 and task 0's columns; in step t, task t + 1's columns,
+// loaded and written into the window, where the first sweep's wave reads them
+// in step t + 1.  Returns false when the chase is abandoned (a bounded wait
+// gave up).
 template <class M>
 __device__ __forceinline__ bool chase_loader(const M &mem, __amdgpu_buffer_rsrc_t rab, int n, int mt,
                                              int s0, int last, double *win, int *sh_abort) {
@@ -1361,18 +1363,14 @@ __device__ __forceinline__ bool chase_loader(const M &mem, __amdgpu_buffer_rsrc_
   lds_barrier();
   if (*sh_abort) return false;
   for (int t = 0; t < steps; ++t) {
-    const bool next = t + 1 < nt;
-    if (next) {
+    if (t + 1 < nt) {
       ok = s == 0 || chase_wait(mem, s - 1, min(t + 3, ntp));
-      if (ok)
+      if (ok) {
         chase_load(mem, rab, n, mt, s, t + 1, nt, ntp, lane, colv, colx, mail0);
-      else if (lane == 0)
-        *sh_abort = 1;
+        ok = chase_put(mem, n, mt, s, t + 1, nt, ntp, lane, win, colv, colx, mail0);
+      }
+      if (!ok && lane == 0) *sh_abort = 1;
     }
-    lds_barrier();
-    if (*sh_abort) return false;
-    if (next && !chase_put(mem, n, mt, s, t + 1, nt, ntp, lane, win, colv, colx, mail0) && lane == 0)
-      *sh_abort = 1;
     lds_barrier();
     if (*sh_abort) return false;
   }
@@ -1523,32 +1521,25 @@ __device__ __forceinline__ bool chase_sweep(const M &mem, __amdgpu_buffer_rsrc_t
   __builtin_amdgcn_s_waitcnt(0x0F70);
   lds_barrier();  // the loading wave's first columns
   if (*sh_abort) return false;
+  // L, v (in vs), tau and beta of the wave's next task, formed at the end of
+  // the previous step (or, for task 0, at the start of its step)
+  double L[16];
+  double tau = 0.0, beta = 0.0, sc = 0.0;
   for (int t = 0; t < steps; ++t) {
     const int k = t - 2 * j;
     const bool act = k >= 0 && k < nt;
     const int p = s + 1 + B * k;
     const int l = min(B, n - p);
     const int l2 = max(0, min(B, n - p - B));
-    double L[16];
-    double tau = 0.0, beta = 0.0, sc = 0.0;
     // interior tasks (full blocks, the head rule in force) need no masks
     const bool full = k >= 2 && l == B && l2 == B;
     const TaskCols tw(win, p, lane);
-    // ---- phase A: L and its reflector
     if (act) {
-      if (full)
-        task_a<true>(tw, win, s, k, l, lane, L, tau, beta, sc, vs);
-      else
-        task_a<false>(tw, win, s, k, l, lane, L, tau, beta, sc, vs);
-      if (k >= 2 && lane == 0) {
-        wcol(win, p - B, B) = beta;  // the head of L, read by the next wave after the barrier
-        if (stores) mem.st_mail((long)s * mt + k, (unsigned long long)__double_as_longlong(beta));
+      // column s is final once sweep s - 1 has completed task 1, one step ago
+      if (k == 0) {
+        task_a<false>(tw, win, s, 0, l, lane, L, tau, beta, sc, vs);
+        wave_sync();
       }
-    }
-    lds_barrier();
-    if (*sh_abort) return false;
-    // ---- phase B: the two-sided update
-    if (act) {
       if (full)
         task_b<true>(tw, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);
       else
@@ -1605,6 +1596,21 @@ __device__ __forceinline__ bool chase_sweep(const M &mem, __amdgpu_buffer_rsrc_t
           // operations issued since task k - 1's stores and task k's mailbox
           asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
           if (lane == 0) mem.st_prog(s, k);
+        }
+      }
+      // the next task's reflector: its L is the Bn this task just wrote; its
+      // head goes into the window for the next wave before the barrier
+      if (k + 1 < nt) {
+        wave_sync();
+        const int p1 = p + B, l1 = min(B, n - p1), l21 = max(0, min(B, n - p1 - B));
+        const TaskCols t1(win, p1, lane);
+        if (k + 1 >= 2 && l1 == B && l21 == B)
+          task_a<true>(t1, win, s, k + 1, l1, lane, L, tau, beta, sc, vs);
+        else
+          task_a<false>(t1, win, s, k + 1, l1, lane, L, tau, beta, sc, vs);
+        if (k + 1 >= 2 && lane == 0) {
+          wcol(win, p1 - B, B) = beta;
+          if (stores) mem.st_mail((long)s * mt + k + 1, (unsigned long long)__double_as_longlong(beta));
         }
       }
     }

@@ -281,9 +281,9 @@ class Group:
 
     def ready(self):
         """The loading wave's dependencies on the previous group: before the
-        first step, the columns of task 0 and its mailbox head; in phase A of
-        step t, the columns of task t + 1, which it loads then; in phase B,
-        the mailbox head of task t + 1 if the early read missed it."""
+        first step, the columns of task 0 and its mailbox head; in step t, the
+        columns of task t + 1, which it loads and writes into the window then,
+        and that task's mailbox head."""
         if self.s0 == 0:
             return True
         t, nt0 = self.t, self.nt[0]
@@ -292,10 +292,9 @@ class Group:
             return prog >= min(2, self.ntp) and (2 >= self.ntp or (self.s0 - 1, 2) in self.mem.mail)
         if t + 1 >= nt0:
             return True
-        if self.phase == 0:
-            return prog >= min(t + 3, self.ntp)
         k = t + 1
-        return not (k + 2 < self.ntp and self.colv_mail is None and (self.s0 - 1, k + 2) not in self.mem.mail)
+        return prog >= min(k + 2, self.ntp) and \
+            not (k + 2 < self.ntp and (self.s0 - 1, k + 2) not in self.mem.mail)
 
     def load_task(self, k):
         """Wave 0's loads for task k: its columns (all 2B diagonals, one
@@ -328,6 +327,9 @@ class Group:
             self.win.barrier()
             self.t = 0
             return
+        # one barrier per step: a compute wave forms the reflector of its next
+        # task right after its update (its L is the Bn just written), and the
+        # reflector of its first task at the start of that task's step
         actors = [(j, self.t - 2 * j) for j in range(len(self.sweeps))]
         actors = [(j, k) for j, k in actors if 0 <= k < self.nt[j]]
         if self.t + 1 < self.nt[0]:
@@ -337,20 +339,18 @@ class Group:
         for i in order:
             j, k = actors[i]
             if j == LOADER:
-                if self.phase == 0:
-                    self.load_task(k)
-                else:
-                    self.put_task(k)
+                self.load_task(k)
+                self.put_task(k)
             else:
-                (self.phase_a if self.phase == 0 else self.phase_b)(j, k)
+                if k == 0:
+                    self.phase_a(j, 0)
+                self.phase_b(j, k)
+                if k + 1 < self.nt[j]:
+                    self.phase_a(j, k + 1)
         self.win.barrier()
-        if self.phase == 0:
-            self.phase = 1
-        else:
-            self.phase = 0
-            self.t += 1
-            if self.t >= self.steps:
-                self.done = True
+        self.t += 1
+        if self.t >= self.steps:
+            self.done = True
 
     def loader_prologue(self):
         """The loading wave before the first step: sweep column s0 and the
@@ -408,7 +408,7 @@ class Group:
             for r in range(l):
                 L[r, 0] = W.get(j, s, 1 + r)
         else:
-            # written as Bn by this wave's previous task
+            # written as Bn by this wave's previous task, just before
             for c in range(B):
                 for r in range(l):
                     L[r, c] = W.get(j, p - B + c, B + r - c)
