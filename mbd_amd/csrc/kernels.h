@@ -79,7 +79,8 @@ constexpr int kSbrLdb = 2 * kSbrB;  // band storage: diagonals 0 .. 2B-1 of each
 constexpr int kSbrMaxLev = 8;
 constexpr int kSbrMaxSplit = 16;   // column splits of the symmetric product
 constexpr int kSbrChaseWG = 256;   // workgroups of the bulge-chasing kernel
-constexpr int kSbrChaseWaves = 4;  // bulge-chasing waves per workgroup: one loads, the others run a sweep each
+constexpr int kSbrChaseSweeps = 3;  // bulge-chasing sweeps per workgroup, one compute wave each
+constexpr int kSbrChaseWaves = kSbrChaseSweeps + 2;  // and one wave loading, one storing
 struct SbrWork {
   double *Vn, *Tn, *Rn;  // TSQR nodes: reflectors (kSbrLeaf x B, row-major), T, R
   int *cnt;              // per-node arrival counters

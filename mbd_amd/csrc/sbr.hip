@@ -1182,32 +1182,34 @@ __global__ __launch_bounds__(256) void k_sbr_band(int n, const double *__restric
 // task's L is the Bn the wave has just written) and writes the head into the
 // window then, before the barrier after which wave j + 1 reads it.
 // The waves share a ring buffer of band columns in LDS (the window: WC
-// columns of 2B diagonals; the group's columns in flight span at most
-// 2 B CS - CS + 2).  A band column crosses between workgroups once per CS
-// sweeps:
-//  * a loading wave brings the columns of the group's first sweep into the
-//    window, one task ahead: it loads task k + 1's columns once the previous
-//    group's last sweep has completed task k + 2 (progress counter), and the
-//    head of that sweep's task k + 3 from a mailbox.  Memory holds that
-//    sweep's values only inside its coverage (diagonals 0 .. 2B - 2 - q of
-//    the task's column q); the loading wave zeroes the rest, which the band
-//    holds as zeros after that sweep but memory may hold as an older group's
-//    value.  It has no arithmetic to do, so the loads stay in flight across
-//    a whole step without taking registers from the compute waves;
-//  * the group's last sweep stores what its task wrote and publishes task
-//    k - 1 once every memory operation it issued before task k's stores has
-//    completed: a wave's memory operations complete in order, so
-//    s_waitcnt vmcnt(#task k stores) is that condition, and the stores of
-//    one task stay in flight while the next step computes.
-// Groups are claimed in increasing order from a counter (flag[2]), so every
-// group a workgroup waits for has been claimed by a running workgroup.
+// columns of 2B diagonals; the group's columns in use span at most
+// 2 B CS - CS + 2 + B, the loading wave's next task's B columns on top).  A
+// band column crosses between workgroups once per CS sweeps, in through a
+// loading wave and out through a storing wave, which have no arithmetic to
+// do: their memory operations and waits stay off the compute waves' path and
+// out of their registers, and the loads do not wait behind the stores (a
+// wave's memory operations complete in order).
+//  * It brings the columns of the group's first sweep into the window, one
+//    task ahead: it loads task k + 1's columns once the previous group's last
+//    sweep has completed task k + 2 (progress counter), and the head of that
+//    sweep's task k + 3 from a mailbox.  Memory holds that sweep's values only
+//    inside its coverage (diagonals 0 .. 2B - 2 - q of the task's column q);
+//    the loading wave zeroes the rest, which the band holds as zeros after
+//    that sweep but memory may hold as an older group's value.
+//  * In the step after the group's last sweep has run task k, the storing
+//    wave stores what that task made final: the previous block's L columns
+//    and the diagonals 0 .. B - 1 - q of the task's column q (the lower half
+//    of D; the rest of those columns is the task's Bn, which the next task is
+//    updating in the same step and which is stored as that task's L), waits
+//    for them (s_waitcnt vmcnt(0)) and publishes task k (progress k + 1).
+//    The mailbox with the head of each task goes out from the compute wave.
 constexpr int CW = kSbrChaseWaves;
-constexpr int CS = CW - 1;
-constexpr int WC = 2 * B * CW;
+constexpr int CS = kSbrChaseSweeps;
+constexpr int WC = 2 * B * (CS + 1);
 constexpr int PW = 2 * B + 2;  // window column pitch: T[c][r - c] reads hit distinct banks
 static_assert((WC & (WC - 1)) == 0, "window size must be a power of two");
-static_assert(CS >= 2 && 2 * B * CS - CS + 2 + B <= WC,
-              "the window must hold the columns in flight and the next task's");
+static_assert(CS >= 2 && 2 * B * CS - CS + 2 + 2 * B <= WC,
+              "the window must hold the columns in use and the next task's");
 static_assert(CS <= B, "a group's sweep columns must lie in its first task's block");
 
 __device__ inline void wave_sync() {
@@ -1253,6 +1255,7 @@ __device__ inline double lane_value(double v, int src) {
 // atomics above (sc1: past the XCD's L2 to the device's coherence point).
 // Offsets outside the buffer read as zero and write nothing.
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
 constexpr int kSc1 = 16;
 
 __device__ inline __amdgpu_buffer_rsrc_t band_rsrc(double *AB, int n) {
@@ -1263,6 +1266,13 @@ __device__ inline void ld2(__amdgpu_buffer_rsrc_t r, int e, double &a, double &b
   a = __longlong_as_double((long long)(((unsigned long long)v[1] << 32) | v[0]));
   b = __longlong_as_double((long long)(((unsigned long long)v[3] << 32) | v[2]));
 }
+__device__ inline void st1(__amdgpu_buffer_rsrc_t r, int e, double a) {
+  const unsigned long long x = (unsigned long long)__double_as_longlong(a);
+  const v2u v = {(unsigned)x, (unsigned)(x >> 32)};
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, e * 8, 0, kSc1);
+}
+// An element offset past any band: a buffer store there writes nothing.
+constexpr int kNoStore = 1 << 27;
 __device__ inline void st2(__amdgpu_buffer_rsrc_t r, int e, double a, double b) {
   const unsigned long long x = (unsigned long long)__double_as_longlong(a);
   const unsigned long long y = (unsigned long long)__double_as_longlong(b);
@@ -1289,8 +1299,10 @@ __device__ inline void chase_load(const M &mem, __amdgpu_buffer_rsrc_t rab, int 
 }
 
 // Task k's loaded columns into the window, restricted to the previous
-// sweep's coverage, with its head from the mailbox.  False if the mailbox
-// wait gave up.
+// sweep's coverage, with its head from the mailbox.  All B columns are
+// written, at the last task also those past column n - 1 (loaded as zeros):
+// their slots belong to the task, and a write without a lane condition keeps
+// the wave unsplit.  False if the mailbox wait gave up.
 template <class M>
 __device__ inline bool chase_put(const M &mem, int n, int mt, int s, int k, int nt, int ntp, int lane,
                                  double *win, const double (&colv)[B], double colx,
@@ -1306,31 +1318,81 @@ __device__ inline bool chase_put(const M &mem, int n, int mt, int s, int k, int 
       return false;
   }
   const int lc = (k + 1 == nt) ? n - p : min(B, n - p);
-  const int sr = lane & 31, sh = lane >> 5;
+  const int sr = lane & 31, sh = lane >> 5, d0 = 2 * sr;
 #pragma unroll
   for (int jj = 0; jj < B / 2; ++jj) {
     const int q = 2 * jj + sh;
-    double v[2] = {colv[2 * jj], colv[2 * jj + 1]};
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int d = 2 * sr + e;
-      if (s > 0) {
-        if (q == B - 1) {
-          if (d > B) v[e] = 0.0;
-          else if (d == B && need_mail) v[e] = bm;  // the previous sweep's next head
-        } else if (d >= max(1, 2 * B - 1 - q)) {
-          v[e] = 0.0;
-        }
-      }
-    }
-    if (q < lc) {
-      double *w = &wcol(win, p + q, 2 * sr);
-      w[0] = v[0];
-      w[1] = v[1];
-    }
+    // diagonals below lim of column q hold the previous sweep's values
+    const int lim = s == 0 ? 2 * B : (q == B - 1 ? B + 1 : max(1, 2 * B - 1 - q));
+    double v0 = d0 < lim ? colv[2 * jj] : 0.0;
+    const double v1 = d0 + 1 < lim ? colv[2 * jj + 1] : 0.0;
+    if (jj == B / 2 - 1) v0 = (need_mail && q == B - 1 && d0 == B) ? bm : v0;  // the previous sweep's next head
+    double *w = &wcol(win, p + q, d0);
+    w[0] = v0;
+    w[1] = v1;
   }
   if (lc > B) wcol(win, p + B, lane) = (s > 0 && lane >= B - 1) ? 0.0 : colx;
   return true;
+}
+
+// The stores of task k of the group's last sweep s, made in the step after
+// the task's, from the window: at task 0 the group's sweep columns, else the
+// previous block's L columns p - B + q at diagonals B - q .. 2B - 1 - q (their
+// head too: the next group takes the head from the mailbox and stores its
+// own value later); then diagonals 0 .. B - 1 - q of the task's column q, at
+// the last task all of 0 .. B - 1 and the one row of its Bn.  16 bytes per
+// lane: four band columns per instruction; a pair that straddles the end of
+// a column's range stores its first element alone (one instruction for all:
+// a lane has at most one such pair), and a lane with nothing to store points
+// past the band, so no lane condition splits the wave.
+template <class M>
+__device__ inline void chase_store(const M &mem, __amdgpu_buffer_rsrc_t rab, int n, int s0, int s, int k,
+                                   int nt, int lane, double *win) {
+  const int p = s + 1 + B * k, l = min(B, n - p), l2 = max(0, min(B, n - p - B));
+  const bool lastk = k + 1 == nt;
+  const int m2 = 2 * (lane & 15), q4 = lane >> 4;
+  if (k == 0) {
+    for (int cc = s0; cc <= s; ++cc)
+      if (lane < (cc < s ? 2 : 1 + l)) mem.st((long)cc * LDB + lane, wcol(win, cc, lane));
+  } else {
+    double lv[B / 4][2];
+#pragma unroll
+    for (int jj = 0; jj < B / 4; ++jj) {
+      const int qq = 4 * jj + q4;
+      const double *w = &wcol(win, p - B + qq, B - qq + m2);
+      lv[jj][0] = w[0];
+      lv[jj][1] = w[1];
+    }
+#pragma unroll
+    for (int jj = 0; jj < B / 4; ++jj) {
+      const int qq = 4 * jj + q4;
+      st2(rab, (p - B + qq) * LDB + B - qq + m2, lv[jj][0], lv[jj][1]);
+    }
+  }
+  double dv[B / 4][2];
+#pragma unroll
+  for (int jj = 0; jj < B / 4; ++jj) {
+    const int qq = 4 * jj + q4;
+    const double *w = &wcol(win, p + qq, m2);
+    dv[jj][0] = w[0];
+    dv[jj][1] = w[1];
+  }
+  // a lane's straddling pair (column qq = B - 1 - m2, before the last task)
+  const int qs = B - 1 - m2;
+  double xs = 0.0;
+#pragma unroll
+  for (int jj = 0; jj < B / 4; ++jj) {
+    const int qq = 4 * jj + q4;
+    const int top = lastk ? B : B - qq;  // diagonals below top are final
+    st2(rab, (qq < l && m2 + 1 < top) ? (p + qq) * LDB + m2 : kNoStore, dv[jj][0], dv[jj][1]);
+    xs = (qq == qs) ? dv[jj][0] : xs;
+  }
+  if (!lastk) st1(rab, ((qs & 3) == q4 && qs < l) ? (p + qs) * LDB + m2 : kNoStore, xs);
+  if (lastk) {
+    // Bn has at most one row at a sweep's last task
+    if (l2 > 0 && lane < l) mem.st((long)(p + lane) * LDB + B - lane, wcol(win, p + lane, B - lane));
+    if (p + l == n - 1 && lane == 0) mem.st((long)(n - 1) * LDB, wcol(win, n - 1, 0));
+  }
 }
 
 __device__ inline int chase_steps(int n, int s0, int last) {
@@ -1345,7 +1407,9 @@ __device__ inline int chase_steps(int n, int s0, int last) {
 template <class M>
 __device__ __forceinline__ bool chase_loader(const M &mem, __amdgpu_buffer_rsrc_t rab, int n, int mt,
                                              int s0, int last, double *win, int *sh_abort) {
-  const int lane = threadIdx.x & 63;
+  const int lane0 = threadIdx.x & 63;
+  int lane = lane0;
+  asm volatile("" : "+v"(lane));
   const int s = s0;
   const int nt = (n - 3 - s) / B + 1;
   const int ntp = s0 > 0 ? (n - 2 - s0) / B + 1 : 0;  // tasks of sweep s0 - 1
@@ -1363,6 +1427,10 @@ __device__ __forceinline__ bool chase_loader(const M &mem, __amdgpu_buffer_rsrc_
   lds_barrier();
   if (*sh_abort) return false;
   for (int t = 0; t < steps; ++t) {
+    // the lane index as a value of this step: the coverage masks are then
+    // formed inside the loop instead of held in (spilled) scalar registers
+    int lane = lane0;
+    asm volatile("" : "+v"(lane));
     if (t + 1 < nt) {
       ok = s == 0 || chase_wait(mem, s - 1, min(t + 3, ntp));
       if (ok) {
@@ -1371,6 +1439,33 @@ __device__ __forceinline__ bool chase_loader(const M &mem, __amdgpu_buffer_rsrc_
       }
       if (!ok && lane == 0) *sh_abort = 1;
     }
+    lds_barrier();
+    if (*sh_abort) return false;
+  }
+  return true;
+}
+
+// The storing wave of the group s0 .. s0 + last: in step t the stores of the
+// last sweep's task of step t - 1, then its progress; one more round after
+// the last step.  Returns false when the chase is abandoned.
+template <class M>
+__device__ __forceinline__ bool chase_storer(const M &mem, __amdgpu_buffer_rsrc_t rab, int n, int s0,
+                                             int last, double *win, int *sh_abort) {
+  const int lane0 = threadIdx.x & 63;
+  const int sl = s0 + last, ntl = (n - 3 - sl) / B + 1;
+  const int steps = chase_steps(n, s0, last);
+  lds_barrier();  // the loading wave's first columns
+  if (*sh_abort) return false;
+  for (int t = 0; t <= steps; ++t) {
+    int lane = lane0;
+    asm volatile("" : "+v"(lane));
+    const int ks = t - 1 - 2 * last;
+    if (ks >= 0 && ks < ntl) {
+      chase_store(mem, rab, n, s0, sl, ks, ntl, lane, win);
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      if (lane == 0) mem.st_prog(sl, ks + 1);
+    }
+    if (t == steps) break;
     lds_barrier();
     if (*sh_abort) return false;
   }
@@ -1389,12 +1484,18 @@ struct TaskCols {
   __device__ double *L(int i) const { return col(c - B) + B + 16 * h + i - c; }
   __device__ double *Dlo(int i) const { return col(c) + 16 * h + i - c; }
   __device__ double *Dup(int i) const { return col(16 * h + i) + c - 16 * h - i; }
+  // D at (r, c) through the symmetry, Dlo for r >= c and Dup otherwise, with
+  // no branch: the column is min(r, c), the diagonal |r - c|
+  __device__ double *Dsym(int i) const {
+    const int r = 16 * h + i, a = min(r, c);
+    return col(a) + (max(r, c) - a);
+  }
   __device__ double *Bn(int i) const { return col(16 * h + i) + B + c - 16 * h - i; }
 };
 
 // Phase A of task k of sweep s: L in column form (k = 0: column s) and the
-// reflector of its first column, v into vs.  FULL: an interior task (k >= 2,
-// full blocks), which needs no masks.
+// reflector of its first column, v into vs.  FULL: a task with full blocks
+// after the first, which needs no masks.
 template <bool FULL>
 __device__ __forceinline__ void task_a(const TaskCols &tc, double *win, int s, int k, int l,
                                        int lane, double (&L)[16], double &tau, double &beta,
@@ -1434,7 +1535,9 @@ __device__ __forceinline__ void task_a(const TaskCols &tc, double *win, int s, i
 // rows 16 h .. 16 h + 15 of column c of L and D and columns 16 h .. 16 h + 15
 // of row c of Bn.  Window reads are unconditional (every address is a valid
 // slot) with the value masked; a write a lane must not make goes to its own
-// dummy slot (mine), so no branch splits the wave.
+// dummy slot (mine), so no branch splits the wave.  FULL: full blocks, no
+// masks; at k = 0 L is column s, else the task's L columns without their
+// head (written with the reflector).
 template <bool FULL>
 __device__ __forceinline__ void task_b(const TaskCols &tc, double *win, int s, int k, int l,
                                        int l2, int lane, double (&L)[16], double tau, double beta,
@@ -1445,7 +1548,7 @@ __device__ __forceinline__ void task_b(const TaskCols &tc, double *win, int s, i
   for (int i = 0; i < 16; ++i) {
     const int r = 16 * h + i;
     // D is exactly symmetric: both triangles come from the same elements
-    const double dv = *((r >= c) ? tc.Dlo(i) : tc.Dup(i));
+    const double dv = *tc.Dsym(i);
     const double bv = *tc.Bn(i);  // Bn[c][r]
     D[i] = (FULL || (r < l && c < l)) ? dv : 0.0;
     Bn[i] = (FULL || (c < l2 && r < l)) ? bv : 0.0;
@@ -1490,34 +1593,34 @@ __device__ __forceinline__ void task_b(const TaskCols &tc, double *win, int s, i
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     const int r = 16 * h + i;
-    if (!FULL && k == 0) {
-      *((c == 0 && r < l) ? &wcol(win, s, 1 + r) : mine) = L[i];
+    if (k == 0) {
+      *((c == 0 && (FULL || r < l)) ? &wcol(win, s, 1 + r) : mine) = L[i];
     } else {
-      *(((FULL || r < l) && !((FULL || k >= 2) && r == 0 && c == 0)) ? tc.L(i) : mine) = L[i];
+      *(((FULL || r < l) && !(r == 0 && c == 0)) ? tc.L(i) : mine) = L[i];
     }
     *((r >= c && (FULL || (r < l && c < l))) ? tc.Dlo(i) : mine) = D[i];
     *((FULL || (c < l2 && r < l)) ? tc.Bn(i) : mine) = Bn[i];
   }
 }
 
-// Compute wave j of the group s0 .. s0 + last (sweep s0 + j); stores: it is
-// the group's last sweep.  Every task writes its Bn to the window, where the
-// next task reads it back as L in column form.  Returns false when the chase
-// is abandoned.
+// Compute wave j of the group s0 .. s0 + last (sweep s0 + j); mail: it is
+// the group's last sweep, whose heads the next group reads from the
+// mailboxes.  Every task writes its Bn to the window, where the next task
+// reads it back as L in column form.  Returns false when the chase is
+// abandoned.
 template <class M>
 __device__ __forceinline__ bool chase_sweep(const M &mem, __amdgpu_buffer_rsrc_t rab, int n, int mt,
-                                            int s0, int last, int j, bool stores, double *win,
+                                            int s0, int last, int j, bool mail, double *win,
                                             double *vs, double *ys, double *dmy, int *sh_abort) {
-  const int lane = threadIdx.x & 63;
+  const int lane0 = threadIdx.x & 63;
   const int s = s0 + j;
   const int nt = j <= last ? (n - 3 - s) / B + 1 : 0;
   const int steps = chase_steps(n, s0, last);
-  double *const mine = dmy + lane;
-  // Nothing of this wave is in flight here (the storing wave's last task
-  // waited for its stores).  Saying so keeps the compiler, which merges the
-  // loading wave's code paths into this one's, from waiting for phantom
-  // loads inside the step loop, where the storing wave's stores must stay in
-  // flight: s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15).
+  // Waiting here for this wave's memory operations (at most the previous
+  // group's mailbox stores) tells the compiler, which merges the loading
+  // wave's code paths into this one's, that nothing is in flight, so that it
+  // does not wait for phantom loads inside the step loop:
+  // s_waitcnt vmcnt(0) expcnt(7) lgkmcnt(15).
   __builtin_amdgcn_s_waitcnt(0x0F70);
   lds_barrier();  // the loading wave's first columns
   if (*sh_abort) return false;
@@ -1526,13 +1629,18 @@ __device__ __forceinline__ bool chase_sweep(const M &mem, __amdgpu_buffer_rsrc_t
   double L[16];
   double tau = 0.0, beta = 0.0, sc = 0.0;
   for (int t = 0; t < steps; ++t) {
+    // the lane index as a value of this step, so that the compiler does not
+    // hoist the lane masks out of the loop into (spilled) scalar registers
+    int lane = lane0;
+    asm volatile("" : "+v"(lane));
+    double *const mine = dmy + lane;
     const int k = t - 2 * j;
     const bool act = k >= 0 && k < nt;
     const int p = s + 1 + B * k;
     const int l = min(B, n - p);
     const int l2 = max(0, min(B, n - p - B));
-    // interior tasks (full blocks, the head rule in force) need no masks
-    const bool full = k >= 2 && l == B && l2 == B;
+    // tasks with full blocks need no masks
+    const bool full = l == B && l2 == B;
     const TaskCols tw(win, p, lane);
     if (act) {
       // column s is final once sweep s - 1 has completed task 1, one step ago
@@ -1544,73 +1652,19 @@ __device__ __forceinline__ bool chase_sweep(const M &mem, __amdgpu_buffer_rsrc_t
         task_b<true>(tw, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);
       else
         task_b<false>(tw, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);
-      if (stores) {
-        // stores from the window, 16 bytes per lane: four band columns per
-        // instruction; every value is read before the first store is issued
-        wave_sync();
-        const int m2 = 2 * (lane & 15), q4 = lane >> 4;
-        const bool lastk = k + 1 == nt;
-        if (k == 0) {
-          for (int cc = s0; cc <= s; ++cc)
-            if (lane < (cc < s ? 2 : 1 + l)) mem.st((long)cc * LDB + lane, wcol(win, cc, lane));
-        } else {
-          // the previous block's columns p - B + qq at diagonals B - qq + r
-          double lv[B / 4][2];
-#pragma unroll
-          for (int jj = 0; jj < B / 4; ++jj) {
-            const int qq = 4 * jj + q4;
-            const double *w = &wcol(win, p - B + qq, B - qq + m2);
-            lv[jj][0] = w[0];
-            lv[jj][1] = w[1];
-          }
-#pragma unroll
-          for (int jj = 0; jj < B / 4; ++jj) {
-            const int qq = 4 * jj + q4;
-            st2(rab, (p - B + qq) * LDB + B - qq + m2, lv[jj][0], lv[jj][1]);
-          }
-        }
-        // diagonals 0 .. B - 1 of the task's columns: the lower half of D
-        // (the rest is Bn, stored again as the next task's L)
-        double dv[B / 4][2];
-#pragma unroll
-        for (int jj = 0; jj < B / 4; ++jj) {
-          const int qq = 4 * jj + q4;
-          const double *w = &wcol(win, p + qq, m2);
-          dv[jj][0] = w[0];
-          dv[jj][1] = w[1];
-        }
-#pragma unroll
-        for (int jj = 0; jj < B / 4; ++jj) {
-          const int qq = 4 * jj + q4;
-          if (qq < l) st2(rab, (p + qq) * LDB + m2, dv[jj][0], dv[jj][1]);
-        }
-        if (lastk) {
-          // Bn has at most one row at a sweep's last task
-          if (l2 > 0 && lane < l)
-            mem.st((long)(p + lane) * LDB + B - lane, wcol(win, p + lane, B - lane));
-          if (p + l == n - 1 && lane == 0) mem.st((long)(n - 1) * LDB, wcol(win, n - 1, 0));
-          asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-          if (lane == 0) mem.st_prog(s, k + 1);
-        } else if (k >= 1) {
-          // the 2 x B / 4 store instructions above are the only memory
-          // operations issued since task k - 1's stores and task k's mailbox
-          asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
-          if (lane == 0) mem.st_prog(s, k);
-        }
-      }
       // the next task's reflector: its L is the Bn this task just wrote; its
       // head goes into the window for the next wave before the barrier
       if (k + 1 < nt) {
         wave_sync();
         const int p1 = p + B, l1 = min(B, n - p1), l21 = max(0, min(B, n - p1 - B));
         const TaskCols t1(win, p1, lane);
-        if (k + 1 >= 2 && l1 == B && l21 == B)
+        if (l1 == B && l21 == B)
           task_a<true>(t1, win, s, k + 1, l1, lane, L, tau, beta, sc, vs);
         else
           task_a<false>(t1, win, s, k + 1, l1, lane, L, tau, beta, sc, vs);
-        if (k + 1 >= 2 && lane == 0) {
+        if (lane == 0) {
           wcol(win, p1 - B, B) = beta;
-          if (stores) mem.st_mail((long)s * mt + k + 1, (unsigned long long)__double_as_longlong(beta));
+          if (mail && k + 1 >= 2) mem.st_mail((long)s * mt + k + 1, (unsigned long long)__double_as_longlong(beta));
         }
       }
     }
@@ -1637,11 +1691,17 @@ __global__ __launch_bounds__(64 * CW) void k_sbr_chase(int n, ChaseArgs args, in
     lds_barrier();
     const int s0 = sh_group * CS;
     if (s0 > n - 3) return;
-    const int last = min(CS, n - 2 - s0) - 1;  // the group's last sweep stores
+    const int last = min(CS, n - 2 - s0) - 1;  // the group's last sweep
+    // wave 0 loads and wave CW - 1 stores: with waves placed on the SIMDs in
+    // turn, the two share one SIMD and each compute wave has one to itself
     const int j = wave - 1;
-    const bool ok = wave == 0 ? chase_loader(mem, rab, n, mt, s0, last, win, &sh_abort)
-                              : chase_sweep(mem, rab, n, mt, s0, last, j, j == last, win, vsh[j],
-                                            ysh[j], dmy[j], &sh_abort);
+    bool ok;
+    if (wave == 0)
+      ok = chase_loader(mem, rab, n, mt, s0, last, win, &sh_abort);
+    else if (wave == CW - 1)
+      ok = chase_storer(mem, rab, n, s0, last, win, &sh_abort);
+    else
+      ok = chase_sweep(mem, rab, n, mt, s0, last, j, j == last, win, vsh[j], ysh[j], dmy[j], &sh_abort);
     if (!ok) return;
   }
 }
@@ -1829,7 +1889,7 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
     ProfScope ps(K_SBR_CHASE, s, 8.0 * 3.5 * B * 0.5 * (double)n * n, 12.0 * B * 0.5 * (double)n * n);
     const ChaseArgs ca{w.AB, w.prog, w.mail, w.flag};
     hipLaunchKernelGGL(k_sbr_chase<ChaseAgent>,
-                       dim3(std::min(ceil_div(n - 2, kSbrChaseWaves - 1), kSbrChaseWG)),
+                       dim3(std::min(ceil_div(n - 2, kSbrChaseSweeps), kSbrChaseWG)),
                        dim3(64 * kSbrChaseWaves), 0, s, n, ca, sbr_mail_tasks(n));
     MBD_LAUNCH_CHECK();
   }

@@ -8,9 +8,10 @@ group and one compute wavefront per sweep, in lock step: at step t compute
 wave j runs task t - 2 j of sweep s0 + j.  The waves of a group share a ring
 buffer of band columns in LDS (the window, WC = 2 B (m + 1) columns).  A separate
 loading wave brings the columns of the group's first sweep into the window,
-one task ahead, once the previous group has published them; only the
-group's last sweep stores columns to memory and publishes its progress and
-the heads of its reflectors through mailboxes.
+one task ahead, once the previous group has published them; a storing wave,
+in the step after each task of the group's last sweep, stores what that task
+made final and publishes its progress; the last sweep's wave sends the heads
+of its reflectors through mailboxes.
 
 This model runs the same schedule with numpy on the CPU, with
 * a race detector on the window: within one phase (between two barriers) no
@@ -124,6 +125,7 @@ class Race(AssertionError):
 
 
 LOADER = -1  # actor id of the loading wave
+STORER = -2  # actor id of the storing wave
 
 
 class Window:
@@ -180,9 +182,8 @@ class Window:
 class Memory:
     """Band, progress counters and mailboxes in device memory.  A store stays
     pending (invisible to loads) until it is committed; commits happen at
-    random.  The stores of one wave complete in the order they were issued,
-    which is what lets the storing wave publish a task's progress once the
-    stores issued before its next task's stores are complete."""
+    random, in the order each wave issued its stores.  The loading wave
+    publishes a task's progress once its stores are complete."""
 
     def __init__(self, AB, n, rng, p_commit):
         self.AB = AB.copy()
@@ -244,7 +245,7 @@ class Memory:
 class Group:
     """One workgroup: sweeps s0 .. s0 + m - 1 in lock step."""
 
-    def __init__(self, g, n, B, m, WC, mem, mask=True):
+    def __init__(self, g, n, B, m, WC, mem, mask=True, dmask=True):
         self.n, self.B, self.m = n, B, m
         self.s0 = g * m
         self.sweeps = [s for s in range(self.s0, self.s0 + m) if s <= n - 3]
@@ -262,12 +263,14 @@ class Group:
         self.colv_seq = 0
         self.in_flight = set()
         self.mask = mask
+        self.dmask = dmask
         self.ntp = ntasks(n, B, self.s0 - 1) if self.s0 > 0 else 0
         self.done = False
 
     def retired_below(self):
         """Columns below this are used by no wave of the group from this step
-        on (a wave not yet started will still read its own sweep column)."""
+        on (a wave not yet started will still read its own sweep column, and
+        the storing wave stores the last sweep's task of the previous step)."""
         low = self.n
         for j, s in enumerate(self.sweeps):
             k = max(self.t, 0) - 2 * j
@@ -277,6 +280,9 @@ class Group:
                 low = min(low, s)
             else:
                 low = min(low, sizes(self.n, self.B, s, k)[0] - self.B)
+        ks = self.t - 1 - 2 * self.last
+        if 0 <= ks < self.nt[self.last]:
+            low = min(low, self.s0 if ks == 0 else sizes(self.n, self.B, self.sweeps[-1], ks)[0] - self.B)
         return low
 
     def ready(self):
@@ -303,7 +309,9 @@ class Group:
         n, B, s = self.n, self.B, self.sweeps[0]
         p, l, _ = sizes(n, B, s, k)
         lc = (n - p) if k + 1 == self.nt[0] else l
-        self.colv = [self.mem.AB[p + q, :].copy() for q in range(lc)]
+        # B columns at least; those past n - 1 load as zeros
+        self.colv = [self.mem.AB[p + q, :].copy() if p + q < n else np.zeros(2 * B)
+                     for q in range(max(lc, B))]
         self.mark_load()
         self.colv_mail = self.mem.mail.get((s - 1, k + 2)) if (s > 0 and k + 2 < self.ntp) else None
 
@@ -334,6 +342,9 @@ class Group:
         actors = [(j, k) for j, k in actors if 0 <= k < self.nt[j]]
         if self.t + 1 < self.nt[0]:
             actors.append((LOADER, self.t + 1))
+        ks = self.t - 1 - 2 * self.last  # the last sweep's task of the previous step
+        if 0 <= ks < self.nt[self.last]:
+            actors.append((STORER, ks))
         order = list(range(len(actors)))
         rng.shuffle(order)
         for i in order:
@@ -341,6 +352,10 @@ class Group:
             if j == LOADER:
                 self.load_task(k)
                 self.put_task(k)
+            elif j == STORER:
+                self.store(k)
+                self.mem.complete(self.s0)
+                self.mem.prog[self.sweeps[-1]] = k + 1
             else:
                 if k == 0:
                     self.phase_a(j, 0)
@@ -349,7 +364,7 @@ class Group:
                     self.phase_a(j, k + 1)
         self.win.barrier()
         self.t += 1
-        if self.t >= self.steps:
+        if self.t > self.steps:  # the storing wave's last stores run at t = steps
             self.done = True
 
     def loader_prologue(self):
@@ -368,8 +383,9 @@ class Group:
         self.put_task(0)
 
     def put_task(self, k):
-        """Task k's loaded columns into the window (phase B of the step
-        before the task's), restricted to the previous sweep's coverage."""
+        """Task k's loaded columns into the window (in the step before the
+        task's), restricted to the previous sweep's coverage; at the last task
+        also the columns past n - 1, whose slots the task owns."""
         n, B, s, W, j = self.n, self.B, self.sweeps[0], self.win, LOADER
         p, l, l2 = sizes(n, B, s, k)
         need_mail = s > 0 and k + 2 < self.ntp
@@ -395,7 +411,7 @@ class Group:
                     vals[max(1, 2 * B - 1 - q):] = 0.0
                     used[max(1, 2 * B - 1 - q):] = False
             for d in range(2 * B):
-                if used[d]:
+                if used[d] and p + q < n:
                     self.use(p + q, d)
                 W.put(j, p + q, d, vals[d])
 
@@ -415,10 +431,11 @@ class Group:
         self.L[j] = L
         tau, beta, v = task_reflector(self.L[j], l)
         self.refl[j] = (tau, beta, v)
+        if k >= 1:
+            W.put(j, p - B, B, beta)  # the head (for wave j + 1 from task 2 on)
         if k >= 2:
-            W.put(j, p - B, B, beta)  # the head, for wave j + 1
             if j == self.last:
-                self.mem.store(self.s0, "mail", (s, k), beta)
+                self.mem.store(-1 - self.s0, "mail", (s, k), beta)
 
     def phase_b(self, j, k):
         n, B, s = self.n, self.B, self.sweeps[j]
@@ -434,7 +451,7 @@ class Group:
         else:
             for c in range(B):
                 for r in range(l):
-                    if not (k >= 2 and r == 0 and c == 0):
+                    if not (r == 0 and c == 0):  # the head, written with the reflector
                         W.put(j, p - B + c, B + r - c, L[r, c])
         for c in range(l):
             for r in range(c, l):
@@ -443,20 +460,23 @@ class Group:
             for r in range(l2):
                 W.put(j, p + c, B + r - c, Bn[r, c])
         self.L[j] = None
-        if j == self.last:
-            self.store(j, k, p, l, l2, last)
 
-    def store(self, j, k, p, l, l2, last):
-        """The storing wave: what it writes to memory and when it publishes.
-        Whole 16-byte pairs of diagonals: the previous block's L columns at
-        diagonals B .. 2B - 1 shifted by the column (head included: the next
-        group takes the head from the mailbox and stores its own value later)
-        and diagonals 0 .. B - 1 of the task's columns (the lower half of D;
-        the rest is the task's Bn, stored again as the next task's L)."""
-        n, B, s, W, mem = self.n, self.B, self.sweeps[j], self.win, self.mem
-        rd = lambda c, d: W.get(j, c, d)
+    def store(self, k):
+        """The storing wave's stores of the last sweep's task k, in the step
+        after the task's.  Whole 16-byte pairs of diagonals: the previous
+        block's L columns at diagonals B .. 2B - 1 shifted by the column (head
+        included: the next group takes the head from the mailbox and stores
+        its own value later) and diagonals 0 .. B - 1 - q of the task's
+        column q (the lower half of D: the rest is the task's Bn, which the
+        next task is updating in this step and which it stores as that task's
+        L), a pair straddling the end alone; at the last task every diagonal
+        0 .. B - 1 and its Bn."""
+        n, B, W, mem = self.n, self.B, self.win, self.mem
+        s = self.sweeps[-1]
+        p, l, l2 = sizes(n, B, s, k)
+        last = k + 1 == self.nt[self.last]
+        rd = lambda c, d: W.get(STORER, c, d)
         st = lambda c, d: mem.store(self.s0, "ab", (c, d), rd(c, d))
-        first = mem.seq
         if k == 0:
             for c in range(self.s0, s + 1):
                 for d in range(2 if c < s else 1 + l):
@@ -466,7 +486,8 @@ class Group:
                 for r in range(B):
                     st(p - B + c, B + r - c)
         for c in range(l):
-            for d in range(B):
+            top = B if last or not self.dmask else B - c
+            for d in range(top):
                 st(p + c, d)
         if last:
             for c in range(l):
@@ -474,16 +495,9 @@ class Group:
                     st(p + c, B + r - c)
             if p + l == n - 1:
                 st(n - 1, 0)  # the last diagonal element, outside this task's block
-            mem.complete(self.s0)
-            mem.prog[s] = k + 1
-        elif k >= 1:
-            # everything issued before this task's first store is complete:
-            # tasks 0 .. k - 1 and the mailbox of task k
-            mem.complete(self.s0, first)
-            mem.prog[s] = k
 
 
-def chase_grouped(AB, n, B, m, WC, seed, max_active=4, p_commit=0.3, mask=True):
+def chase_grouped(AB, n, B, m, WC, seed, max_active=4, p_commit=0.3, mask=True, dmask=True):
     rng = np.random.default_rng(seed)
     mem = Memory(AB, n, rng, p_commit)
     ngroups = (n - 2 + m - 1) // m if n >= 3 else 0
@@ -491,7 +505,7 @@ def chase_grouped(AB, n, B, m, WC, seed, max_active=4, p_commit=0.3, mask=True):
     nxt = 0
     while True:
         while nxt < ngroups and sum(not g.done for g in groups) < max_active:
-            g = Group(nxt, n, B, m, WC, mem, mask)
+            g = Group(nxt, n, B, m, WC, mem, mask, dmask)
             groups.append(g)
             nxt += 1
         active = [g for g in groups if not g.done]
@@ -510,6 +524,13 @@ def chase_grouped(AB, n, B, m, WC, seed, max_active=4, p_commit=0.3, mask=True):
     mem.commit_all()
     mem.check_loads()
     return mem.AB
+
+
+def window_cols(B, m):
+    """The kernel's window: the columns in use span 2 B m - m + 2 + B (the
+    storing wave stores the last sweep's previous task), and the loading
+    wave writes the next task's B columns a step ahead."""
+    return max(2 * B * (m + 1), 2 * B * m - m + 2 + 2 * B)
 
 
 def random_band(n, B, seed):
@@ -534,10 +555,7 @@ def test_grouped_chase_matches_sequential(B, m, n):
     AB = band_of(A, B)
     ref = chase_sequential(AB, n, B)
     for seed in range(3 if B < 32 else 1):
-        # the columns in flight span 2 B m - m + 2, and the loading wave
-        # writes the next task's B columns a step ahead: the kernel's window
-        # is 2 B (m + 1) columns
-        got = chase_grouped(AB, n, B, m, 2 * B * (m + 1), seed)
+        got = chase_grouped(AB, n, B, m, window_cols(B, m), seed)
         assert np.array_equal(got[:, 0], ref[:, 0])
         assert np.array_equal(got[:n - 1, 1], ref[:n - 1, 1])
     d, e = ref[:, 0], ref[:n - 1, 1]
@@ -559,7 +577,7 @@ def test_first_wave_must_zero_beyond_coverage():
     wrong = 0
     for seed in range(4):
         try:
-            got = chase_grouped(AB, n, B, m, 2 * B * (m + 1), seed, mask=False)
+            got = chase_grouped(AB, n, B, m, window_cols(B, m), seed, mask=False)
             wrong += not np.array_equal(got[:, 0], ref[:, 0])
         except Race:
             wrong += 1
@@ -571,3 +589,12 @@ def test_window_too_small_is_caught():
     AB = band_of(random_band(n, B, 3), B)
     with pytest.raises(Race):
         chase_grouped(AB, n, B, m, 2 * B * m, 0)
+
+
+def test_store_must_stop_before_the_next_tasks_rows():
+    # Storing diagonals B - q .. B - 1 of the task's column q with its D half
+    # reads the next task's L while that task is updating it.
+    B, m, n = 4, 3, 44
+    AB = band_of(random_band(n, B, 7 + n), B)
+    with pytest.raises(Race):
+        chase_grouped(AB, n, B, m, window_cols(B, m), 0, dmask=False)
