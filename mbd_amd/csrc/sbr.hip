@@ -1536,9 +1536,9 @@ __device__ __forceinline__ void task_a(const TaskCols &tc, double *win, int s, i
 // of row c of Bn.  Window reads are unconditional (every address is a valid
 // slot) with the value masked; a write a lane must not make goes to its own
 // dummy slot (mine), so no branch splits the wave.  FULL: full blocks, no
-// masks; at k = 0 L is column s, else the task's L columns without their
-// head (written with the reflector).
-template <bool FULL>
+// masks.  At k = 0 L is column s (K0, when FULL), else the task's L columns
+// without their head (written with the reflector).
+template <bool FULL, bool K0 = false>
 __device__ __forceinline__ void task_b(const TaskCols &tc, double *win, int s, int k, int l,
                                        int l2, int lane, double (&L)[16], double tau, double beta,
                                        const double *vs, double *ys, double *mine) {
@@ -1593,7 +1593,7 @@ __device__ __forceinline__ void task_b(const TaskCols &tc, double *win, int s, i
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     const int r = 16 * h + i;
-    if (k == 0) {
+    if (FULL ? K0 : k == 0) {
       *((c == 0 && (FULL || r < l)) ? &wcol(win, s, 1 + r) : mine) = L[i];
     } else {
       *(((FULL || r < l) && !(r == 0 && c == 0)) ? tc.L(i) : mine) = L[i];
@@ -1648,7 +1648,9 @@ __device__ __forceinline__ bool chase_sweep(const M &mem, __amdgpu_buffer_rsrc_t
         task_a<false>(tw, win, s, 0, l, lane, L, tau, beta, sc, vs);
         wave_sync();
       }
-      if (full)
+      if (full && k == 0)
+        task_b<true, true>(tw, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);
+      else if (full)
         task_b<true>(tw, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);
       else
         task_b<false>(tw, win, s, k, l, l2, lane, L, tau, beta, vs, ys, mine);
