@@ -617,19 +617,46 @@ __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, doub
       }
 }
 
-// Last of the grid's blocks to arrive sums the partial Grams into G (LDS).
+// Last of the grid's blocks to arrive sums the partial Grams into G (LDS),
+// 16 loads in flight per element.
 __device__ inline bool gram_reduce_last(const double *Gp, int *cnt, Mat32 *G) {
   __shared__ int last;
   if (!arrive_last(cnt, (int)gridDim.x, &last)) return false;
   if (threadIdx.x == 0) *cnt = 0;
-  for (int e = threadIdx.x; e < B * B; e += NT) {
-    double s = 0.0;
-#pragma unroll 8
-    for (int b = 0; b < (int)gridDim.x; ++b) s += ld_coh(&Gp[(long)b * B * B + e]);
-    G[e / B][e % B] = s;
+  const int nb = (int)gridDim.x;
+  double acc[B * B / NT];
+#pragma unroll
+  for (int q = 0; q < B * B / NT; ++q) acc[q] = 0.0;
+  for (int b0 = 0; b0 < nb; b0 += 16) {
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = threadIdx.x + NT * q;
+      double v[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) v[i] = ld_coh(&Gp[(long)min(b0 + i, nb - 1) * B * B + e]);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[q] += (b0 + i < nb) ? v[i] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < B * B / NT; ++q) {
+    const int e = threadIdx.x + NT * q;
+    G[e / B][e % B] = acc[q];
   }
   __syncthreads();
   return true;
+}
+
+// Row `row` of the first B columns of an m x w block P (column-major, lda),
+// zero past m or w.  The loads go to clamped addresses without a lane
+// condition and all issue before the first is used: a load under a lane
+// condition followed by its LDS store is waited for one at a time.
+__device__ inline void load_row(const double *P, long lda, int row, int m, int w, double (&v)[B]) {
+  const int rr = min(row, m - 1);
+#pragma unroll
+  for (int k = 0; k < B; ++k) v[k] = P[rr + (long)min(k, w - 1) * lda];
+#pragma unroll
+  for (int k = 0; k < B; ++k) v[k] = (row < m && k < w) ? v[k] : 0.0;
 }
 }  // namespace
 
@@ -640,8 +667,12 @@ __global__ __launch_bounds__(NT) void k_sbr_cq1(int m, int w, const double *__re
   __shared__ double Ps[B][NT + 1];
   __shared__ Mat32 G[B], R[B];
   const int t = threadIdx.x, row = NT * blockIdx.x + t;
+  {
+    double v[B];
+    load_row(P, lda, row, m, w, v);
 #pragma unroll
-  for (int k = 0; k < B; ++k) Ps[k][t] = (row < m && k < w) ? P[row + (long)k * lda] : 0.0;
+    for (int k = 0; k < B; ++k) Ps[k][t] = v[k];
+  }
   __syncthreads();
   block_gram(Ps, Gp + (long)blockIdx.x * B * B);
   if (!gram_reduce_last(Gp, cnt, G)) return;
@@ -663,9 +694,16 @@ __global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__re
   __shared__ Mat32 R[B], Ri[B], G[B];
   __shared__ double Qs[B][NT + 1];
   const int t = threadIdx.x, row = NT * blockIdx.x + t;
-  for (int e = t; e < B * B; e += NT) R[e / B][e % B] = Rbuf[e];
+  {
+    double v[B], rv[B * B / NT];
 #pragma unroll
-  for (int k = 0; k < B; ++k) Ps[k][t] = (row < m && k < w) ? P[row + (long)k * lda] : 0.0;
+    for (int q = 0; q < B * B / NT; ++q) rv[q] = Rbuf[t + NT * q];
+    load_row(P, lda, row, m, w, v);
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) R[(t + NT * q) / B][(t + NT * q) % B] = rv[q];
+#pragma unroll
+    for (int k = 0; k < B; ++k) Ps[k][t] = v[k];
+  }
   __syncthreads();
   lds_inv_upper(R, Ri);
   // Q1 = P R1^{-1} for the block's rows, to global and to LDS for the Gram
@@ -697,9 +735,20 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
   __shared__ double Qb[B][NT + 1];  // the block's rows of Q1
   __shared__ double sgn[B];
   const int t = threadIdx.x;
-  for (int e = t; e < B * B; e += NT) {
-    R2[e / B][e % B] = Rbuf[B * B + e];
-    Y1[e / B][e % B] = Q1[e / B + (long)(e % B) * ldq];  // top block of Q1
+  {
+    double rv[B * B / NT], yv[B * B / NT];
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q;
+      rv[q] = Rbuf[B * B + e];
+      yv[q] = Q1[e / B + (long)(e % B) * ldq];  // top block of Q1
+    }
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q;
+      R2[e / B][e % B] = rv[q];
+      Y1[e / B][e % B] = yv[q];
+    }
   }
   __syncthreads();
   lds_inv_upper(R2, R2i);
@@ -753,8 +802,12 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
   const int g = NT * blockIdx.x + t;
   // y = q1 My (into V1 and V2), x = q1 Mx, for every row on MFMA; the top
   // block's rows are then replaced by their exact form below
+  {
+    double v[B];
+    load_row(Q1, ldq, g, m, B, v);
 #pragma unroll
-  for (int k = 0; k < B; ++k) Qb[k][t] = (g < m) ? Q1[g + (long)k * ldq] : 0.0;
+    for (int k = 0; k < B; ++k) Qb[k][t] = v[k];
+  }
   __syncthreads();
   rows_mm(Qb, My, V1, ldv, NT * blockIdx.x, m, nullptr);
   rows_mm(Qb, My, V2, ldv, NT * blockIdx.x, m, nullptr);
@@ -778,7 +831,11 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
   if (blockIdx.x == 0) {
     // band block S R, R = R2 R1 (Wm is free again)
     __syncthreads();
-    for (int e = t; e < B * B; e += NT) Y1[e / B][e % B] = Rbuf[e];
+    double rv[B * B / NT];
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) rv[q] = Rbuf[t + NT * q];
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) Y1[(t + NT * q) / B][(t + NT * q) % B] = rv[q];
     __syncthreads();
     lds_mm(Wm, R2, Y1, 1.0);
     if (t < B)
@@ -889,19 +946,23 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
   const bool pend = Vq != nullptr;
   {
     // every load first, then the stores: a store into Yp between the loads
-    // would order each column's loads behind the previous column's store
+    // would order each column's loads behind the previous column's store.
+    // Clamped rows and splits instead of conditions, so that the loads of a
+    // column issue together.
+    const int rr = min(row, m - 1);
     double ys[8], xv[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int c = 8 * cg + j;
-      double s = 0.0;
-      if (row < m) {
+      double v[kSbrMaxSplit];
 #pragma unroll
-        for (int q = 0; q < kSbrMaxSplit; ++q)
-          if (q < nsplit) s += Yp[(long)q * B * ldy + row + (long)c * ldy];
-      }
-      ys[j] = s;
-      xv[j] = (row < m) ? X[row + (long)c * ldx] : 0.0;
+      for (int q = 0; q < kSbrMaxSplit; ++q) v[q] = Yp[(long)min(q, nsplit - 1) * B * ldy + rr + (long)c * ldy];
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < kSbrMaxSplit; ++q) s += (q < nsplit) ? v[q] : 0.0;
+      ys[j] = (row < m) ? s : 0.0;
+      const double x = X[rr + (long)c * ldx];
+      xv[j] = (row < m) ? x : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -929,11 +990,11 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
     for (int pass = 0; pass < 2; ++pass) {
       const double *Q = pass ? Zq : Vq;
       __syncthreads();
+      double qv[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int c = 8 * cg + j;
-        Ys[c][r] = (row < m) ? Q[row + (long)c * ldq] : 0.0;
-      }
+      for (int j = 0; j < 8; ++j) qv[j] = Q[min(row, m - 1) + (long)(8 * cg + j) * ldq];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Ys[8 * cg + j][r] = (row < m) ? qv[j] : 0.0;
       __syncthreads();
       tile(Ys, Xs, Mp + (long)I * NQ + (1 + pass) * B * B);
     }
@@ -948,13 +1009,14 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
 #pragma unroll
     for (int i = 0; i < NQ / 256; ++i) {
       const int e = t + 256 * i;
-      double s = 0.0;
-      if (e < nq) {
+      const int ee = min(e, nq - 1);
+      double v[G];
 #pragma unroll
-        for (int q = 0; q < G; ++q)
-          if (q < gsz) s += ld_coh(&Mp[(long)(grp * G + q) * NQ + e]);
-      }
-      acc[i] = s;
+      for (int q = 0; q < G; ++q) v[q] = ld_coh(&Mp[(long)(grp * G + min(q, gsz - 1)) * NQ + ee]);
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < G; ++q) s += (q < gsz) ? v[q] : 0.0;
+      acc[i] = (e < nq) ? s : 0.0;
     }
 #pragma unroll
     for (int i = 0; i < NQ / 256; ++i)
@@ -968,12 +1030,16 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
 #pragma unroll
   for (int i = 0; i < NQ / 256; ++i) {
     const int e = t + 256 * i;
+    const int ee = min(e, nq - 1);
     double s = 0.0;
-    if (e < nq) {
-#pragma unroll 4
-      for (int g = 0; g < ng; ++g) s += ld_coh(&Mg[(long)g * NQ + e]);
+    for (int g0 = 0; g0 < ng; g0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int g = 0; g < 8; ++g) v[g] = ld_coh(&Mg[(long)min(g0 + g, ng - 1) * NQ + ee]);
+#pragma unroll
+      for (int g = 0; g < 8; ++g) s += (g0 + g < ng) ? v[g] : 0.0;
     }
-    acc[i] = s;
+    acc[i] = (e < nq) ? s : 0.0;
   }
 #pragma unroll
   for (int i = 0; i < NQ / 256; ++i) {
@@ -1015,19 +1081,39 @@ __global__ __launch_bounds__(256) void k_sbr_z(int m, const double *__restrict__
   const int row0 = 64 * blockIdx.x, row = row0 + r;
   const bool pend = Vq != nullptr;
   const int kk = pend ? K : B;
-  for (int e = t; e < kk * B; e += 256) {
-    const int i = e / B, k = e % B;
-    // M row-major at M[0..), GV at M[B*B..), GZ at M[2*B*B..)
-    Cs[i][k] = (i < B) ? 0.5 * M[i * B + k]
-                       : (i < 2 * B ? M[2 * B * B + (i - B) * B + k] : M[B * B + (i - 2 * B) * B + k]);
-  }
+  const int rr = min(row, m - 1);
+  {
+    // loads first (clamped rows, no lane conditions), then the LDS stores
+    double cv[K * B / 256], av[8], qv[8], zv[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int c = 8 * cg + j;
-    As[r][c] = (row < m) ? V[row + (long)c * ldv] : 0.0;
+    for (int q = 0; q < K * B / 256; ++q) {
+      const int e = t + 256 * q, i = e / B, k = e % B;
+      // M row-major at M[0..), GV at M[B*B..), GZ at M[2*B*B..)
+      const long src = (i < B) ? i * B + k : (i < 2 * B ? 2 * B * B + (i - B) * B + k : B * B + (i - 2 * B) * B + k);
+      cv[q] = (e < kk * B) ? M[src] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) av[j] = V[rr + (long)(8 * cg + j) * ldv];
     if (pend) {
-      As[r][B + c] = (row < m) ? Vq[row + (long)c * ldq] : 0.0;
-      As[r][2 * B + c] = (row < m) ? Zq[row + (long)c * ldq] : 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        qv[j] = Vq[rr + (long)(8 * cg + j) * ldq];
+        zv[j] = Zq[rr + (long)(8 * cg + j) * ldq];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < K * B / 256; ++q) {
+      const int e = t + 256 * q, i = e / B, k = e % B;
+      if (e < kk * B) Cs[i][k] = (i < B) ? 0.5 * cv[q] : cv[q];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int c = 8 * cg + j;
+      As[r][c] = (row < m) ? av[j] : 0.0;
+      if (pend) {
+        As[r][B + c] = (row < m) ? qv[j] : 0.0;
+        As[r][2 * B + c] = (row < m) ? zv[j] : 0.0;
+      }
     }
   }
   __syncthreads();
@@ -1038,12 +1124,19 @@ __global__ __launch_bounds__(256) void k_sbr_z(int m, const double *__restrict__
     acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Cs[4 * k4 + lk][lr], acc0, 0, 0, 0);
     acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Cs[4 * k4 + lk][16 + lr], acc1, 0, 0, 0);
   }
+  double y0[4], y1[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int rr = row0 + 16 * cg + lk + 4 * v;
-    if (rr < m) {
-      Z[rr + (long)lr * ldv] = Y[rr + (long)lr * ldy] - acc0[v];
-      Z[rr + (long)(16 + lr) * ldv] = Y[rr + (long)(16 + lr) * ldy] - acc1[v];
+    const int rq = min(row0 + 16 * cg + lk + 4 * v, m - 1);
+    y0[v] = Y[rq + (long)lr * ldy];
+    y1[v] = Y[rq + (long)(16 + lr) * ldy];
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int rq = row0 + 16 * cg + lk + 4 * v;
+    if (rq < m) {
+      Z[rq + (long)lr * ldv] = y0[v] - acc0[v];
+      Z[rq + (long)(16 + lr) * ldv] = y1[v] - acc1[v];
     }
   }
 }
