@@ -94,7 +94,7 @@ struct SbrWork {
   int *flag;             // [0]: bulge chase abandoned (spin limit)
   unsigned long long *mail;  // reflector heads passed between sweeps, n x sbr_mail_tasks(n)
   int *cq;               // CholeskyQR2 panels: [0], [1] arrival counters, [2 + p] path flag of panel p
-  double *cqR;           // R1 and R2 of the current panel, 2 x B x B
+  double *cqR;           // R1, R2, R1^{-1} and R2^{-1} of the current panel, 4 x B x B
   double *VW;            // two panels' factors [V_a | V_b | Z_a | Z_b | V_a | V_b], n x 6B (ld ldv)
   long ldv;
   double *d, *e;         // tridiagonal output

@@ -487,63 +487,51 @@ __device__ inline void lds_mm(Mat32 *C, const Mat32 *A, const Mat32 *Bm, double 
   __syncthreads();
 }
 
-// Ai = A^{-1} for A upper triangular (zeros below).  Thread c < B computes
-// column c in registers, fully unrolled from the bottom row up (static
-// indices, so the loads of A issue ahead of the dependent chain).
-__device__ inline void lds_inv_upper(const Mat32 *A, Mat32 *Ai) {
+// Ai = A^{-1} for A upper triangular (zeros below), by doubling: with the
+// diagonal blocks of size b inverted, each pair of them is completed as
+// [X11 X12; 0 X22] with X12 = -X11 A12 X22, one dot of at most b terms per
+// element and two barriers per level (five levels).  T: scratch.
+__device__ inline void lds_inv_upper(const Mat32 *A, Mat32 *Ai, Mat32 *T) {
   const int t = threadIdx.x;
-  if (t < B) {
-    double col[B], rd[B];
-#pragma unroll
-    for (int i = 0; i < B; ++i) rd[i] = 1.0 / A[i][i];
-#pragma unroll
-    for (int i = B - 1; i >= 0; --i) {
-      double s0 = (i == t) ? 1.0 : 0.0, s1 = 0.0;
-#pragma unroll
-      for (int l = i + 1; l < B; ++l) {
-        if (l & 1)
-          s1 -= A[i][l] * col[l];
-        else
-          s0 -= A[i][l] * col[l];
-      }
-      col[i] = (s0 + s1) * rd[i];
-    }
-#pragma unroll
-    for (int i = 0; i < B; ++i) Ai[i][t] = col[i];
+  for (int e = t; e < B * B; e += NT) {
+    const int i = e / B, j = e % B;
+    Ai[i][j] = (i == j) ? 1.0 / A[i][i] : 0.0;
   }
   __syncthreads();
-}
-
-// Ai = A^{-1} for A unit lower triangular (strict lower part stored, zeros
-// elsewhere); same organisation, top row down.
-__device__ inline void lds_inv_unit_lower(const Mat32 *A, Mat32 *Ai) {
-  const int t = threadIdx.x;
-  if (t < B) {
-    double col[B];
 #pragma unroll
-    for (int i = 0; i < B; ++i) {
-      double s0 = (i == t) ? 1.0 : 0.0, s1 = 0.0;
-#pragma unroll
-      for (int l = 0; l < i; ++l) {
-        if (l & 1)
-          s1 -= A[i][l] * col[l];
-        else
-          s0 -= A[i][l] * col[l];
-      }
-      col[i] = s0 + s1;
+  for (int b = 1; b < B; b *= 2) {
+    const bool act = t < 16 * b;  // B / (2 b) pairs of b x b blocks
+    const int pr = t / (b * b), rem = t % (b * b);
+    const int o = 2 * b * pr, i = o + rem / b, j = o + b + rem % b;
+    if (act) {
+      double s = 0.0;
+      for (int l = o + b; l <= j; ++l) s += A[i][l] * Ai[l][j];  // A12 X22
+      T[i][j] = s;
     }
-#pragma unroll
-    for (int i = 0; i < B; ++i) Ai[i][t] = col[i];
+    __syncthreads();
+    if (act) {
+      double s = 0.0;
+      for (int l = i; l < o + b; ++l) s += Ai[i][l] * T[l][j];  // X11 (A12 X22)
+      Ai[i][j] = -s;
+    }
+    __syncthreads();
   }
-  __syncthreads();
 }
 
-// Upper Cholesky G = R^T R: R (zeros below) from the upper triangle of G, which
-// is overwritten.  False when a pivot is not above tol.  One barrier per step:
-// the trailing update uses the unscaled row k divided by its pivot.
-__device__ inline bool lds_chol32(Mat32 *G, Mat32 *R, double tol) {
+// Upper Cholesky G = R^T R: R (zeros below) from the upper triangle of G,
+// which is overwritten, and Ri = R^{-1}.  False when a pivot is not above
+// tol.  Symmetric elimination in place with one barrier per step (step k
+// reads row k, which it does not write); the same row operations on the
+// identity E give E = L^{-1} for G = L D L^T, and R^{-T} = D^{-1/2} E, so
+// row k of E scaled by the pivot's inverse square root is column k of Ri.
+__device__ inline bool lds_chol32(Mat32 *G, Mat32 *R, Mat32 *Ri, Mat32 *E, double tol) {
   const int t = threadIdx.x;
-  for (int e = t; e < B * B; e += NT) R[e / B][e % B] = 0.0;
+  for (int e = t; e < B * B; e += NT) {
+    const int i = e / B, j = e % B;
+    R[i][j] = 0.0;
+    Ri[i][j] = 0.0;
+    E[i][j] = (i == j) ? 1.0 : 0.0;
+  }
   __syncthreads();
   bool ok = true;
   for (int k = 0; k < B; ++k) {
@@ -551,18 +539,16 @@ __device__ inline bool lds_chol32(Mat32 *G, Mat32 *R, double tol) {
     ok = ok && (d > tol);
     const double dd = (d > tol) ? d : 1.0;
     const double rs = 1.0 / sqrt(dd), rdd = 1.0 / dd;
-    double nv[B * B / NT];
 #pragma unroll
     for (int q = 0; q < B * B / NT; ++q) {
       const int e = t + NT * q, i = e / B, j = e % B;
-      nv[q] = (i > k && j >= i) ? G[i][j] - G[k][i] * G[k][j] * rdd : 0.0;
       if (i == k && j >= k) R[k][j] = G[k][j] * rs;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < B * B / NT; ++q) {
-      const int e = t + NT * q, i = e / B, j = e % B;
-      if (i > k && j >= i) G[i][j] = nv[q];
+      if (i == k && j <= k) Ri[j][k] = E[k][j] * rs;
+      if (i > k) {
+        const double gki = G[k][i];
+        if (j >= i) G[i][j] = G[i][j] - gki * G[k][j] * rdd;
+        if (j <= k) E[i][j] = E[i][j] - (gki * rdd) * E[k][j];
+      }
     }
     __syncthreads();
   }
@@ -660,12 +646,13 @@ __device__ inline void load_row(const double *P, long lda, int row, int m, int w
 }
 }  // namespace
 
-// R1 = chol(P^T P) into Rbuf[0 .. B*B); *fail = 1 if the Cholesky is not clean.
+// R1 = chol(P^T P) into Rbuf[0 .. B*B) and R1^{-1} into Rbuf[2 B*B ..);
+// *fail = 1 if the Cholesky is not clean.
 __global__ __launch_bounds__(NT) void k_sbr_cq1(int m, int w, const double *__restrict__ P, long lda,
                                                 double *__restrict__ Gp, int *cnt,
                                                 double *__restrict__ Rbuf, int *fail) {
   __shared__ double Ps[B][NT + 1];
-  __shared__ Mat32 G[B], R[B];
+  __shared__ Mat32 G[B], R[B], Ri[B];
   const int t = threadIdx.x, row = NT * blockIdx.x + t;
   {
     double v[B];
@@ -678,13 +665,18 @@ __global__ __launch_bounds__(NT) void k_sbr_cq1(int m, int w, const double *__re
   if (!gram_reduce_last(Gp, cnt, G)) return;
   double dmax = 0.0;
   for (int k = 0; k < B; ++k) dmax = fmax(dmax, G[k][k]);
-  const bool ok = lds_chol32(G, R, 1e-10 * dmax) && dmax > 0.0;
-  for (int e = t; e < B * B; e += NT) Rbuf[e] = R[e / B][e % B];
+  Mat32 *const E = reinterpret_cast<Mat32 *>(&Ps[0][0]);  // the panel rows are no longer needed
+  const bool ok = lds_chol32(G, R, Ri, E, 1e-10 * dmax) && dmax > 0.0;
+  for (int e = t; e < B * B; e += NT) {
+    Rbuf[e] = R[e / B][e % B];
+    Rbuf[2 * B * B + e] = Ri[e / B][e % B];
+  }
   if (t == 0) *fail = ok ? 0 : 1;
 }
 
-// Q1 = P R1^{-1} (rows into Q1), R2 = chol(Q1^T Q1) into Rbuf[B*B .. 2 B*B);
-// *fail = 1 if Q1^T Q1 is not close to I or its Cholesky is not clean.
+// Q1 = P R1^{-1} (rows into Q1), R2 = chol(Q1^T Q1) into Rbuf[B*B .. 2 B*B)
+// and R2^{-1} into Rbuf[3 B*B ..); *fail = 1 if Q1^T Q1 is not close to I or
+// its Cholesky is not clean.
 __global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__restrict__ P, long lda,
                                                 double *__restrict__ Q1, long ldq,
                                                 double *__restrict__ Gp, int *cnt,
@@ -697,15 +689,14 @@ __global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__re
   {
     double v[B], rv[B * B / NT];
 #pragma unroll
-    for (int q = 0; q < B * B / NT; ++q) rv[q] = Rbuf[t + NT * q];
+    for (int q = 0; q < B * B / NT; ++q) rv[q] = Rbuf[2 * B * B + t + NT * q];  // R1^{-1}
     load_row(P, lda, row, m, w, v);
 #pragma unroll
-    for (int q = 0; q < B * B / NT; ++q) R[(t + NT * q) / B][(t + NT * q) % B] = rv[q];
+    for (int q = 0; q < B * B / NT; ++q) Ri[(t + NT * q) / B][(t + NT * q) % B] = rv[q];
 #pragma unroll
     for (int k = 0; k < B; ++k) Ps[k][t] = v[k];
   }
   __syncthreads();
-  lds_inv_upper(R, Ri);
   // Q1 = P R1^{-1} for the block's rows, to global and to LDS for the Gram
   rows_mm(Ps, Ri, Q1, ldq, NT * blockIdx.x, m, Qs);
   __syncthreads();
@@ -716,8 +707,12 @@ __global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__re
   for (int e = t; e < B * B; e += NT)
     dev = fmax(dev, fabs(G[e / B][e % B] - (e / B == e % B ? 1.0 : 0.0)));
   dev = block_max(dev);
-  const bool ok = (dev < 0.1) && lds_chol32(G, R, 0.5);
-  for (int e = t; e < B * B; e += NT) Rbuf[B * B + e] = R[e / B][e % B];
+  Mat32 *const E = reinterpret_cast<Mat32 *>(&Ps[0][0]);  // the panel rows are no longer needed
+  const bool ok = (dev < 0.1) && lds_chol32(G, R, Ri, E, 0.5);
+  for (int e = t; e < B * B; e += NT) {
+    Rbuf[B * B + e] = R[e / B][e % B];
+    Rbuf[3 * B * B + e] = Ri[e / B][e % B];
+  }
   if (t == 0) *fail = ok ? 0 : 1;
 }
 
@@ -736,22 +731,23 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
   __shared__ double sgn[B];
   const int t = threadIdx.x;
   {
-    double rv[B * B / NT], yv[B * B / NT];
+    double rv[B * B / NT], riv[B * B / NT], yv[B * B / NT];
 #pragma unroll
     for (int q = 0; q < B * B / NT; ++q) {
       const int e = t + NT * q;
       rv[q] = Rbuf[B * B + e];
+      riv[q] = Rbuf[3 * B * B + e];
       yv[q] = Q1[e / B + (long)(e % B) * ldq];  // top block of Q1
     }
 #pragma unroll
     for (int q = 0; q < B * B / NT; ++q) {
       const int e = t + NT * q;
       R2[e / B][e % B] = rv[q];
+      R2i[e / B][e % B] = riv[q];
       Y1[e / B][e % B] = yv[q];
     }
   }
   __syncthreads();
-  lds_inv_upper(R2, R2i);
   lds_mm(Wm, Y1, R2i, 1.0);  // top block of Q
   if (t < B) sgn[t] = (Wm[t][t] >= 0.0) ? -1.0 : 1.0;
   __syncthreads();
@@ -760,32 +756,43 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
     Wm[r][k] = Wm[r][k] * sgn[k] - (r == k ? 1.0 : 0.0);
   }
   __syncthreads();
-  // LU without pivoting (|pivot| >= 1 by the choice of signs)
-  for (int kk = 0; kk < B - 1; ++kk) {
-    const double rpiv = 1.0 / Wm[kk][kk];
-    double lm[B * B / NT];
+  // LU without pivoting (|pivot| >= 1 by the choice of signs) and Yi = L^{-1}
+  // by the same row operations on the identity, one barrier per step: step
+  // kk reads row kk and column kk and writes neither; the multipliers of
+  // column kk go in at step kk + 1, after every thread has read the column.
+  for (int e = t; e < B * B; e += NT) Yi[e / B][e % B] = (e / B == e % B) ? 1.0 : 0.0;
+  __syncthreads();
+  {
+    double lprev[B * B / NT];
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) lprev[q] = 0.0;
+    for (int kk = 0; kk < B - 1; ++kk) {
+      const double rpiv = 1.0 / Wm[kk][kk];
+#pragma unroll
+      for (int q = 0; q < B * B / NT; ++q) {
+        const int e = t + NT * q, r = e / B, c = e % B;
+        if (kk > 0 && r > kk - 1 && c == kk - 1) Wm[r][c] = lprev[q];
+        const double lm = Wm[r][kk] * rpiv;
+        if (r > kk && c > kk) Wm[r][c] -= lm * Wm[kk][c];
+        if (r > kk && c <= kk) Yi[r][c] -= lm * Yi[kk][c];
+        lprev[q] = lm;
+      }
+      __syncthreads();
+    }
 #pragma unroll
     for (int q = 0; q < B * B / NT; ++q) {
       const int e = t + NT * q, r = e / B, c = e % B;
-      lm[q] = Wm[r][kk] * rpiv;
-      if (r > kk && c > kk) Wm[r][c] -= lm[q] * Wm[kk][c];
+      if (r > B - 2 && c == B - 2) Wm[r][c] = lprev[q];
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < B * B / NT; ++q) {
-      const int e = t + NT * q, r = e / B, c = e % B;
-      if (r > kk && c == kk) Wm[r][c] = lm[q];
-    }
-    __syncthreads();
   }
+  __syncthreads();
   for (int e = t; e < B * B; e += NT) {
     const int r = e / B, k = e % B;
     U[r][k] = (k >= r) ? Wm[r][k] : 0.0;
     Y1[r][k] = (k < r) ? Wm[r][k] : 0.0;
   }
   __syncthreads();
-  lds_inv_upper(U, Ui);
-  lds_inv_unit_lower(Y1, Yi);
+  lds_inv_upper(U, Ui, Tf);
   // Tf = -U Y1^{-T}
   for (int e = t; e < B * B; e += NT) {
     const int i = e / B, k = e % B;
@@ -1831,7 +1838,7 @@ SbrSizes sbr_sizes(int n) {
   z.prog = (size_t)n;
   z.mail = (size_t)n * sbr_mail_tasks(n);
   z.cq = (size_t)n / B + 4;  // two arrival counters, then one path flag per panel
-  z.cqR = 2 * (size_t)B * B;
+  z.cqR = 4 * (size_t)B * B;
   return z;
 }
 
