@@ -503,15 +503,25 @@ __device__ inline void lds_inv_upper(const Mat32 *A, Mat32 *Ai, Mat32 *T) {
     const bool act = t < 16 * b;  // B / (2 b) pairs of b x b blocks
     const int pr = t / (b * b), rem = t % (b * b);
     const int o = 2 * b * pr, i = o + rem / b, j = o + b + rem % b;
+    // fixed trip counts with zero factors past the triangles, so that the
+    // LDS reads of a dot issue together
     if (act) {
       double s = 0.0;
-      for (int l = o + b; l <= j; ++l) s += A[i][l] * Ai[l][j];  // A12 X22
+#pragma unroll
+      for (int l = 0; l < b; ++l) {
+        const double a = A[i][o + b + l], x = Ai[o + b + l][j];  // A12 X22
+        s += (o + b + l <= j) ? a * x : 0.0;
+      }
       T[i][j] = s;
     }
     __syncthreads();
     if (act) {
       double s = 0.0;
-      for (int l = i; l < o + b; ++l) s += Ai[i][l] * T[l][j];  // X11 (A12 X22)
+#pragma unroll
+      for (int l = 0; l < b; ++l) {
+        const double x = Ai[i][o + l], y = T[o + l][j];  // X11 (A12 X22)
+        s += (o + l >= i) ? x * y : 0.0;
+      }
       Ai[i][j] = -s;
     }
     __syncthreads();
@@ -534,21 +544,31 @@ __device__ inline bool lds_chol32(Mat32 *G, Mat32 *R, Mat32 *Ri, Mat32 *E, doubl
   }
   __syncthreads();
   bool ok = true;
+  // thread t: column j = t % B, rows i0 + 8 q
+  constexpr int NQ = B * B / NT;
+  const int j = t % B, i0 = t / B;
   for (int k = 0; k < B; ++k) {
-    const double d = G[k][k];
+    // every LDS read of the step first, unconditionally (a read under a
+    // lane condition is waited for before the next one is issued)
+    const double d = G[k][k], gkj = G[k][j], ekj = E[k][j];
+    double gki[NQ], gij[NQ], eij[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int i = i0 + (NT / B) * q;
+      gki[q] = G[k][i];
+      gij[q] = G[i][j];
+      eij[q] = E[i][j];
+    }
     ok = ok && (d > tol);
     const double dd = (d > tol) ? d : 1.0;
     const double rs = 1.0 / sqrt(dd), rdd = 1.0 / dd;
 #pragma unroll
-    for (int q = 0; q < B * B / NT; ++q) {
-      const int e = t + NT * q, i = e / B, j = e % B;
-      if (i == k && j >= k) R[k][j] = G[k][j] * rs;
-      if (i == k && j <= k) Ri[j][k] = E[k][j] * rs;
-      if (i > k) {
-        const double gki = G[k][i];
-        if (j >= i) G[i][j] = G[i][j] - gki * G[k][j] * rdd;
-        if (j <= k) E[i][j] = E[i][j] - (gki * rdd) * E[k][j];
-      }
+    for (int q = 0; q < NQ; ++q) {
+      const int i = i0 + (NT / B) * q;
+      if (i == k && j >= k) R[k][j] = gkj * rs;
+      if (i == k && j <= k) Ri[j][k] = ekj * rs;
+      if (i > k && j >= i) G[i][j] = gij[q] - gki[q] * gkj * rdd;
+      if (i > k && j <= k) E[i][j] = eij[q] - (gki[q] * rdd) * ekj;
     }
     __syncthreads();
   }
@@ -573,10 +593,10 @@ __device__ inline void block_gram(const double (*Ps)[NT + 1], double *Gp) {
 
 // Rows of the block times a B x B matrix on MFMA: out[r][c] = sum_k Ls[k][r] Mt[k][c]
 // for the block's NT rows (Ls[col][row] in LDS), wave v owning rows 64 v .. 64 v + 63.
-// Results go to out (global, ld, rows row0.. below m) and, if Os is non-null,
-// to Os[col][row] in LDS.
+// Results go to out (global, ld, rows row0.. below m), to out2 as well if
+// non-null and, if Os is non-null, to Os[col][row] in LDS.
 __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, double *out, long ld,
-                               int row0, int m, double (*Os)[NT + 1]) {
+                               int row0, int m, double (*Os)[NT + 1], double *out2 = nullptr) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 15, lk = lane >> 4;
   v4d acc[4][2];
 #pragma unroll
@@ -599,6 +619,7 @@ __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, doub
       for (int v = 0; v < 4; ++v) {
         const int rl = 64 * wv + 16 * rt + lk + 4 * v, c = 16 * ct + lr;
         if (row0 + rl < m) out[row0 + rl + (long)c * ld] = acc[rt][ct][v];
+        if (out2 && row0 + rl < m) out2[row0 + rl + (long)c * ld] = acc[rt][ct][v];
         if (Os) Os[c][rl] = acc[rt][ct][v];
       }
 }
@@ -766,22 +787,34 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
     double lprev[B * B / NT];
 #pragma unroll
     for (int q = 0; q < B * B / NT; ++q) lprev[q] = 0.0;
+    constexpr int NQ = B * B / NT;
+    const int c = t % B, r0 = t / B;  // thread t: column c, rows r0 + 8 q
     for (int kk = 0; kk < B - 1; ++kk) {
-      const double rpiv = 1.0 / Wm[kk][kk];
+      // every LDS read of the step first, unconditionally
+      const double piv = Wm[kk][kk], wkc = Wm[kk][c], ykc = Yi[kk][c];
+      double wrk[NQ], wrc[NQ], yrc[NQ];
 #pragma unroll
-      for (int q = 0; q < B * B / NT; ++q) {
-        const int e = t + NT * q, r = e / B, c = e % B;
+      for (int q = 0; q < NQ; ++q) {
+        const int r = r0 + (NT / B) * q;
+        wrk[q] = Wm[r][kk];
+        wrc[q] = Wm[r][c];
+        yrc[q] = Yi[r][c];
+      }
+      const double rpiv = 1.0 / piv;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int r = r0 + (NT / B) * q;
         if (kk > 0 && r > kk - 1 && c == kk - 1) Wm[r][c] = lprev[q];
-        const double lm = Wm[r][kk] * rpiv;
-        if (r > kk && c > kk) Wm[r][c] -= lm * Wm[kk][c];
-        if (r > kk && c <= kk) Yi[r][c] -= lm * Yi[kk][c];
+        const double lm = wrk[q] * rpiv;
+        if (r > kk && c > kk) Wm[r][c] = wrc[q] - lm * wkc;
+        if (r > kk && c <= kk) Yi[r][c] = yrc[q] - lm * ykc;
         lprev[q] = lm;
       }
       __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < B * B / NT; ++q) {
-      const int e = t + NT * q, r = e / B, c = e % B;
+    for (int q = 0; q < NQ; ++q) {
+      const int r = r0 + (NT / B) * q;
       if (r > B - 2 && c == B - 2) Wm[r][c] = lprev[q];
     }
   }
@@ -816,8 +849,7 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
     for (int k = 0; k < B; ++k) Qb[k][t] = v[k];
   }
   __syncthreads();
-  rows_mm(Qb, My, V1, ldv, NT * blockIdx.x, m, nullptr);
-  rows_mm(Qb, My, V2, ldv, NT * blockIdx.x, m, nullptr);
+  rows_mm(Qb, My, V1, ldv, NT * blockIdx.x, m, nullptr, V2);
   rows_mm(Qb, Mx, X, ldx, NT * blockIdx.x, m, nullptr);
   __syncthreads();  // the product's stores to the top rows land before their replacement
   if (g < m && g < B) {
