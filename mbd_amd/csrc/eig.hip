@@ -581,7 +581,12 @@ __device__ inline int sturm_count(int n, const double *__restrict__ d,
   if (fabs(q) < pivmin) q = -pivmin;
   cnt += (q < 0.0);
   for (int i = 1; i < n; ++i) {
-    q = (d[i] - x) - e2[i - 1] / q;
+    // e2 / q through the hardware reciprocal and one Newton step (within an
+    // ulp or two of the rounded quotient; the recurrence is bound by the
+    // division, whose correctly rounded form takes several times as long)
+    double r = __builtin_amdgcn_rcp(q);
+    r = fma(r, fma(-q, r, 1.0), r);
+    q = (d[i] - x) - e2[i - 1] * r;
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += (q < 0.0);
   }
