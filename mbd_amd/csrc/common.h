@@ -54,6 +54,10 @@ void gemm_nt(hipStream_t s, bool lower, int m, int n, int k, double alpha,
 void gemm_nt(hipStream_t s, bool lower, int m, int n, int k, float alpha,
              const float *A, int lda, long sA, const float *B, int ldb,
              long sB, float beta, float *C, int ldc, long sC, int batch);
+// C = alpha (A1 B1^T + A2 B2^T) + beta C for n <= 32 columns and k <= 128
+// (A2, B2 may be null); column-major, A1 and A2 with lda, B1 and B2 with ldb.
+void gemm_ts2(hipStream_t s, int m, int n, int k, double alpha, const double *A1, const double *B1,
+              const double *A2, const double *B2, int lda, int ldb, double beta, double *C, int ldc);
 
 // FP32 C = alpha op(A) op(B)^T + beta C (single matrix) where op(A) is m x k
 // and op(B) is n x k: ta reads A as k x m, tb reads B as k x n.

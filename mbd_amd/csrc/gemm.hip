@@ -156,6 +156,88 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(
       }
 }
 
+// Tall-skinny FP64 update C = alpha (A1 B1^T + A2 B2^T) + beta C for at most
+// TS_N columns and TS_K terms of each product (A2, B2 may be null).  A
+// 128 x 128 tile would leave three quarters of its MFMA work on zero columns
+// and give a tall matrix only m / 128 workgroups; here a workgroup owns 64
+// rows of C, each wave 16 rows by the 32 columns as two 16 x 16 MFMA blocks,
+// computed transposed (the B columns as the MFMA's first operand) so that the
+// 16 lanes of a group hold 16 consecutive rows of one column of C.  K goes
+// through LDS in chunks of TS_KC (25 KB of LDS, so that a workgroup fits on
+// a CU beside two workgroups of the square-tile kernel running on another
+// stream), the next chunk's loads issued before the current chunk's MFMAs.
+constexpr int TS_M = 64, TS_N = 32, TS_K = 128, TS_KC = 32;
+
+__global__ __launch_bounds__(256) void k_gemm_ts(int m, int n, int k, double alpha,
+                                                 const double *__restrict__ A1,
+                                                 const double *__restrict__ B1,
+                                                 const double *__restrict__ A2,
+                                                 const double *__restrict__ B2, int lda, int ldb,
+                                                 double beta, double *__restrict__ C, int ldc) {
+  __shared__ double As[TS_KC][TS_M + 1];  // As[kk][row]
+  __shared__ double Bs[TS_KC][TS_N + 1];  // Bs[kk][col]
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, lr = lane & 15, lk = lane >> 4;
+  const int row0 = blockIdx.x * TS_M;
+  const int r = lane, rr = min(row0 + r, m - 1);
+  v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  const int nch = (k + TS_KC - 1) / TS_KC, nterm = A2 ? 2 : 1, nstep = nterm * nch;
+  // A chunk: element e = t + 256 i is row e % 64, column w + 4 i; B chunk:
+  // row e % 32, column e / 32; clamped addresses, zeros past m, n and k
+  double ra[TS_M * TS_KC / 256], rb[TS_N * TS_KC / 256];
+  auto gload = [&](int st) {
+    const int term = st / nch, k0 = (st % nch) * TS_KC;
+    const double *A = term ? A2 : A1, *Bm = term ? B2 : B1;
+#pragma unroll
+    for (int i = 0; i < TS_M * TS_KC / 256; ++i) {
+      const int kk = k0 + w + 4 * i;
+      const double v = A[rr + (long)min(kk, k - 1) * lda];
+      ra[i] = (kk < k && row0 + r < m) ? v : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < TS_N * TS_KC / 256; ++i) {
+      const int e = t + 256 * i, c = e & (TS_N - 1), kk = k0 + (e >> 5);
+      const double v = Bm[min(c, n - 1) + (long)min(kk, k - 1) * ldb];
+      rb[i] = (c < n && kk < k) ? v : 0.0;
+    }
+  };
+  gload(0);
+  for (int st = 0; st < nstep; ++st) {
+    __syncthreads();  // the previous chunk's LDS reads are done
+#pragma unroll
+    for (int i = 0; i < TS_M * TS_KC / 256; ++i) As[w + 4 * i][r] = ra[i];
+#pragma unroll
+    for (int i = 0; i < TS_N * TS_KC / 256; ++i) {
+      const int e = t + 256 * i;
+      Bs[e >> 5][e & (TS_N - 1)] = rb[i];
+    }
+    __syncthreads();
+    if (st + 1 < nstep) gload(st + 1);
+#pragma unroll
+    for (int q = 0; q < TS_KC / 4; ++q) {
+      const double a = As[4 * q + lk][16 * w + lr];
+      const double b0 = Bs[4 * q + lk][lr], b1 = Bs[4 * q + lk][16 + lr];
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a, acc1, 0, 0, 0);
+    }
+  }
+  // transposed blocks: lane (lk, lr) holds C[row 16 w + lr][col lk + 4 v] (+16)
+  const int row = row0 + 16 * w + lr;
+  if (row < m) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int c0 = lk + 4 * v, c1 = 16 + lk + 4 * v;
+      if (c0 < n) {
+        double *pc = C + row + (long)c0 * ldc;
+        *pc = alpha * acc0[v] + (beta != 0.0 ? beta * *pc : 0.0);
+      }
+      if (c1 < n) {
+        double *pc = C + row + (long)c1 * ldc;
+        *pc = alpha * acc1[v] + (beta != 0.0 ? beta * *pc : 0.0);
+      }
+    }
+  }
+}
+
 template <typename T>
 void launch(hipStream_t s, bool lower, int m, int n, int k, T alpha,
             const T *A, int lda, long sA, const T *B, int ldb, long sB,
@@ -183,9 +265,21 @@ void gemm_f32(hipStream_t s, int ta, int tb, int m, int n, int k, float alpha, c
   launch<float>(s, false, m, n, k, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1, ta, tb);
 }
 
+void gemm_ts2(hipStream_t s, int m, int n, int k, double alpha, const double *A1, const double *B1,
+              const double *A2, const double *B2, int lda, int ldb, double beta, double *C, int ldc) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(k_gemm_ts, dim3(ceil_div(m, TS_M)), dim3(256), 0, s, m, n, k, alpha, A1, B1, A2,
+                     B2, lda, ldb, beta, C, ldc);
+  MBD_LAUNCH_CHECK();
+}
+
 void gemm_nt(hipStream_t s, bool lower, int m, int n, int k, double alpha,
              const double *A, int lda, long sA, const double *B, int ldb,
              long sB, double beta, double *C, int ldc, long sC, int batch) {
+  if (!lower && batch == 1 && n <= TS_N && k >= 1 && k <= TS_K) {
+    gemm_ts2(s, m, n, k, alpha, A, B, nullptr, nullptr, lda, ldb, beta, C, ldc);
+    return;
+  }
   launch<double>(s, lower, m, n, k, alpha, A, lda, sA, B, ldb, sB, beta, C,
                  ldc, sC, batch);
 }

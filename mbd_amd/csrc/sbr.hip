@@ -1917,10 +1917,8 @@ void sbr_tridiagonalize(hipStream_t s, int n, double *A, long lda, SbrWork &w) {
     if (second) {
       // the pending update on this panel's columns (rows c.., the first panel's rows 0..)
       ProfScope ps(K_TRD_SYR2K, s, 0.0, 4.0 * B * (double)(n - c) * pw);
-      gemm_nt(s, false, n - c, pw, B, -1.0, Va, (int)ldv, 0, Za, (int)ldv, 0, 1.0,
-              A + c + (long)c * lda, (int)lda, 0, 1);
-      gemm_nt(s, false, n - c, pw, B, -1.0, Za, (int)ldv, 0, Va, (int)ldv, 0, 1.0,
-              A + c + (long)c * lda, (int)lda, 0, 1);
+      gemm_ts2(s, n - c, pw, B, -1.0, Va, Za, Za, Va, (int)ldv, (int)ldv, 1.0, A + c + (long)c * lda,
+               (int)lda);
     }
     int *fail = w.cq + 2 + panel;  // 0: CholeskyQR2 factored the panel, 1: TSQR does
     {

@@ -90,13 +90,17 @@ __global__ __launch_bounds__(W) void k_symv3_strip(int n, const T *__restrict__ 
   // the tile's elements are loaded one tile ahead of their use, so that a
   // thread always has a whole tile of loads in flight
   T nxt[KT];
+  // clamped addresses and a select instead of a lane condition on the load:
+  // the tile's loads then issue without a branch each (the upper part of a
+  // diagonal tile is read and discarded)
   auto load_tile = [&](int R0) {
     const int row = R0 + r;
-    const T *arow = M + row + (long)J0 * lda;
+    const T *arow = M + min(row, n - 1) + (long)J0 * lda;
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
       const int c = cg + CG * k, col = J0 + c;
-      nxt[k] = (row < n && col < n && row >= col) ? arow[(long)c * lda] : (T)0;
+      const T v = arow[(long)min(c, n - 1 - J0) * lda];
+      nxt[k] = (row < n && col < n && row >= col) ? v : (T)0;
     }
   };
   if (rbeg < rend) load_tile(rbeg);

@@ -12,8 +12,11 @@ def _e():
     return engine
 
 
+# n <= 32 with k <= 128 takes the tall-skinny kernel (64-row workgroups, K in
+# one LDS piece), the rest the 128 x 128 tiles
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (16, 16, 4), (128, 128, 16), (130, 70, 33),
-                                   (300, 257, 129)])
+                                   (300, 257, 129), (1000, 32, 128), (777, 17, 33), (65, 32, 1),
+                                   (300, 20, 200)])
 def test_gemm_nt_fp64(m, n, k):
     rng = np.random.default_rng(m * 7 + n)
     A = rng.standard_normal((m, k))
