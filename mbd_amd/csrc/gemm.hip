@@ -139,21 +139,40 @@ __global__ __launch_bounds__(NT, 2) void k_gemm_nt(
     __syncthreads();
   }
 
+  // C in four row blocks of 16 x 64 per wave: a block's loads of C all issue
+  // before its first store (a load after a store to C, which may alias it,
+  // would otherwise wait for the previous element's round trip)
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 4; ++i) {
+    T cv[4][4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int r = row0 + wm * 64 + i * 16 + (Mfma<T>::kSwap ? lr : Mfma<T>::row(lk, v));
         const int c = col0 + wn * 64 + j * 16 + (Mfma<T>::kSwap ? Mfma<T>::row(lk, v) : lr);
-        if (r < m && c < n && (!LOWER || r >= c)) {
-          T *pc = C + r + (long)c * ldc;
-          T val = alpha * acc[i][j][v];
-          if (beta != T(0)) val += beta * (*pc);
-          *pc = val;
-        }
+        const bool in = r < m && c < n && (!LOWER || r >= c);
+        cv[j][v] = (beta != T(0) && in) ? C[r + (long)c * ldc] : T(0);
       }
+    // every value before the first store, each in its own register (a store
+    // reads its data late, so a register it holds is not reused before it
+    // completes)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        cv[j][v] = alpha * acc[i][j][v] + beta * cv[j][v];
+        asm volatile("" : "+v"(cv[j][v]));  // computed here, not sunk into the stores' branches
+      }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = row0 + wm * 64 + i * 16 + (Mfma<T>::kSwap ? lr : Mfma<T>::row(lk, v));
+        const int c = col0 + wn * 64 + j * 16 + (Mfma<T>::kSwap ? Mfma<T>::row(lk, v) : lr);
+        if (r < m && c < n && (!LOWER || r >= c)) C[r + (long)c * ldc] = cv[j][v];
+      }
+  }
 }
 
 // Tall-skinny FP64 update C = alpha (A1 B1^T + A2 B2^T) + beta C for at most
