@@ -593,6 +593,9 @@ __device__ inline void block_gram(const double (*Ps)[NT + 1], double *Gp) {
 
 // Rows of the block times a B x B matrix on MFMA: out[r][c] = sum_k Ls[k][r] Mt[k][c]
 // for the block's NT rows (Ls[col][row] in LDS), wave v owning rows 64 v .. 64 v + 63.
+// Each 16 x 16 block is computed transposed (Mt as the MFMA's first operand),
+// so that the 16 lanes of a group hold 16 consecutive rows of one column and
+// every store to the column-major outputs covers 128 contiguous bytes.
 // Results go to out (global, ld, rows row0.. below m), to out2 as well if
 // non-null and, if Os is non-null, to Os[col][row] in LDS.
 __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, double *out, long ld,
@@ -607,8 +610,8 @@ __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, doub
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt) {
       const double a = Ls[4 * s4 + lk][64 * wv + 16 * rt + lr];
-      acc[rt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[rt][0], 0, 0, 0);
-      acc[rt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[rt][1], 0, 0, 0);
+      acc[rt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a, acc[rt][0], 0, 0, 0);
+      acc[rt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a, acc[rt][1], 0, 0, 0);
     }
   }
 #pragma unroll
@@ -617,7 +620,7 @@ __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, doub
     for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int rl = 64 * wv + 16 * rt + lk + 4 * v, c = 16 * ct + lr;
+        const int rl = 64 * wv + 16 * rt + lr, c = 16 * ct + lk + 4 * v;
         if (row0 + rl < m) out[row0 + rl + (long)c * ld] = acc[rt][ct][v];
         if (out2 && row0 + rl < m) out2[row0 + rl + (long)c * ld] = acc[rt][ct][v];
         if (Os) Os[c][rl] = acc[rt][ct][v];
@@ -946,19 +949,20 @@ __global__ __launch_bounds__(256) void k_sbr_symm(int m, const double *__restric
     for (int kk = 0; kk < SY_K; kk += 4) {
       const double a = As[cur][kk + lk][16 * wv + lr];
       const double b0 = Xs[cur][kk + lk][lr], b1 = Xs[cur][kk + lk][16 + lr];
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+      // transposed blocks: lane (lk, lr) accumulates Y[row 16 wv + lr][col lk + 4 v]
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a, acc1, 0, 0, 0);
     }
     if (it + 1 < nk) sstore(cur ^ 1);
     __syncthreads();
   }
   double *Y = Yp + (long)blockIdx.y * B * ldy;
+  const int row = r0 + 16 * wv + lr;
+  if (row < m) {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int row = r0 + 16 * wv + lk + 4 * v;
-    if (row < m) {
-      Y[row + (long)lr * ldy] = acc0[v];
-      Y[row + (long)(16 + lr) * ldy] = acc1[v];
+    for (int v = 0; v < 4; ++v) {
+      Y[row + (long)(lk + 4 * v) * ldy] = acc0[v];
+      Y[row + (long)(16 + lk + 4 * v) * ldy] = acc1[v];
     }
   }
 }
@@ -1160,22 +1164,22 @@ __global__ __launch_bounds__(256) void k_sbr_z(int m, const double *__restrict__
   v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
   for (int k4 = 0; k4 < kk / 4; ++k4) {
     const double a = As[16 * cg + lr][4 * k4 + lk];
-    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Cs[4 * k4 + lk][lr], acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Cs[4 * k4 + lk][16 + lr], acc1, 0, 0, 0);
+    // transposed blocks: lane (lk, lr) accumulates row 16 cg + lr, column lk + 4 v
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Cs[4 * k4 + lk][lr], a, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Cs[4 * k4 + lk][16 + lr], a, acc1, 0, 0, 0);
   }
+  const int rq = row0 + 16 * cg + lr, rc = min(rq, m - 1);
   double y0[4], y1[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int rq = min(row0 + 16 * cg + lk + 4 * v, m - 1);
-    y0[v] = Y[rq + (long)lr * ldy];
-    y1[v] = Y[rq + (long)(16 + lr) * ldy];
+    y0[v] = Y[rc + (long)(lk + 4 * v) * ldy];
+    y1[v] = Y[rc + (long)(16 + lk + 4 * v) * ldy];
   }
+  if (rq < m) {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int rq = row0 + 16 * cg + lk + 4 * v;
-    if (rq < m) {
-      Z[rq + (long)lr * ldv] = y0[v] - acc0[v];
-      Z[rq + (long)(16 + lr) * ldv] = y1[v] - acc1[v];
+    for (int v = 0; v < 4; ++v) {
+      Z[rq + (long)(lk + 4 * v) * ldv] = y0[v] - acc0[v];
+      Z[rq + (long)(16 + lk + 4 * v) * ldv] = y1[v] - acc1[v];
     }
   }
 }
