@@ -14,11 +14,12 @@
 //   B operand: lane l -> B^T[k l/16][col l%16]
 //   D (FP32):  lane l -> D[row 4*(l/16)+v][col l%16], v = 0..3
 //   D (FP64):  lane l -> D[row (l/16)+4*v][col l%16], v = 0..3
-// FP32 computes each 16 x 16 block transposed (the B tile as the MFMA's A
+// Both compute each 16 x 16 block transposed (the B tile as the MFMA's A
 // operand), so the 16 lanes of a group hold 16 consecutive rows of one column
-// of C and every epilogue access covers whole 64-byte column segments instead
-// of 16 scattered 16-byte pieces (measured faster for the FP32 Cholesky
-// updates; the FP64 updates measured slower that way and keep the direct form).
+// of C and every epilogue access covers whole column segments (64 bytes in
+// FP32, 128 in FP64) instead of 16 scattered pieces; the epilogue issues a
+// 16-row block's loads of C together and stores only after every value of the
+// block is formed.
 #include "common.h"
 
 namespace mbd {
@@ -30,7 +31,7 @@ struct Mfma;
 template <>
 struct Mfma<double> {
   using acc_t = v4d;
-  static constexpr bool kSwap = false;
+  static constexpr bool kSwap = true;
   // row of accumulator element v held by lane group g = lane / 16
   __device__ static int row(int g, int v) { return g + 4 * v; }
   __device__ static acc_t mma(double a, double b, acc_t c) {
