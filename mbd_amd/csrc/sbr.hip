@@ -465,6 +465,23 @@ __device__ inline double block_max(double v) {
   return r;
 }
 
+// 1 / sqrt(x) and 1 / x for x > 0 from the hardware estimates refined by two
+// Newton steps each (a few ulp; the correctly rounded square root and
+// division are long dependent chains on the factorisations' critical path)
+__device__ inline double rsq_fast(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h, y * y, 1.5);
+  y = y * fma(-h, y * y, 1.5);
+  return y;
+}
+__device__ inline double rcp_fast(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
 // true in the last of `total` workgroups to call it on cnt
 __device__ inline bool arrive_last(int *cnt, int total, int *last) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -561,7 +578,7 @@ __device__ inline bool lds_chol32(Mat32 *G, Mat32 *R, Mat32 *Ri, Mat32 *E, doubl
     }
     ok = ok && (d > tol);
     const double dd = (d > tol) ? d : 1.0;
-    const double rs = 1.0 / sqrt(dd), rdd = 1.0 / dd;
+    const double rs = rsq_fast(dd), rdd = rs * rs;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int i = i0 + (NT / B) * q;
@@ -803,7 +820,7 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
         wrc[q] = Wm[r][c];
         yrc[q] = Yi[r][c];
       }
-      const double rpiv = 1.0 / piv;
+      const double rpiv = rcp_fast(piv);  // |piv| >= 1
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const int r = r0 + (NT / B) * q;
