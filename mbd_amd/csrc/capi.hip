@@ -636,7 +636,24 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
                  h->sbrMail.p, h->sbrCq.p, h->sbrCqR.p, h->VW.p, nm, h->dd.p, h->ee.p};
     if (sbr_lookahead()) {
       h->sbrVW2.ensure((size_t)nm * 6 * kSbrB);
-      if (!h->stream2) MBD_HIP(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+      if (!h->stream2) {
+        // The look-ahead SYR2K fills every CU it may use with two workgroups
+        // of 64 KB of LDS each, and the panel kernels of the main stream (up
+        // to 157 KB) then wait for whole CUs to drain: its stream leaves
+        // kSbrFreeCus CUs to the main stream (a stream's CU mask holds for
+        // eager launches only, not for a replayed graph).
+        int ncu = 0;
+        MBD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device));
+        const char *fe = getenv("MBD_SBR_FREE_CUS");
+        const int nfree = fe ? atoi(fe) : kSbrFreeCus;
+        if (nfree > 0 && nfree < ncu) {
+          std::vector<uint32_t> mask((ncu + 31) / 32, 0u);
+          for (int c = 0; c < ncu - nfree; ++c) mask[c / 32] |= 1u << (c % 32);
+          MBD_HIP(hipExtStreamCreateWithCUMask(&h->stream2, (uint32_t)mask.size(), mask.data()));
+        } else {
+          MBD_HIP(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+        }
+      }
       if (!h->sbr_fork) MBD_HIP(hipEventCreateWithFlags(&h->sbr_fork, hipEventDisableTiming));
       if (!h->sbr_join) MBD_HIP(hipEventCreateWithFlags(&h->sbr_join, hipEventDisableTiming));
       sw.VW2 = h->sbrVW2.p;
@@ -651,8 +668,13 @@ void run_energy(mbd_handle *h, const double *alpha_eff_d, const double *c6_d,
     else
       tridiagonalize(s, (int)nm, h->C.p, nm, wk);
   };
+  // The one-stage reduction (thousands of small launches) replays a captured
+  // graph; the two-stage one launches eagerly, which measured faster (with
+  // the look-ahead stream's CU mask in force).  MBD_TRD_GRAPH=0|1 forces
+  // either way.
   const char *g = getenv("MBD_TRD_GRAPH");
-  if (h->prof_on || (g && g[0] == '0')) {
+  const bool graph = g ? g[0] != '0' : method != 2;
+  if (h->prof_on || !graph) {
     reduce();
   } else {
     const std::vector<const void *> key = {(const void *)(intptr_t)nm, (const void *)(intptr_t)method,

@@ -79,6 +79,7 @@ constexpr int kSbrLdb = 2 * kSbrB;  // band storage: diagonals 0 .. 2B-1 of each
 constexpr int kSbrMaxLev = 8;
 constexpr int kSbrMaxSplit = 16;   // column splits of the symmetric product
 constexpr int kSbrChaseWG = 256;   // workgroups of the bulge-chasing kernel
+constexpr int kSbrFreeCus = 32;     // CUs the band reduction's look-ahead stream leaves to the main stream
 constexpr int kSbrChaseSweeps = 3;  // bulge-chasing sweeps per workgroup, one compute wave each
 constexpr int kSbrChaseWaves = kSbrChaseSweeps + 2;  // and one wave loading, one storing
 struct SbrWork {
