@@ -614,7 +614,8 @@ __device__ inline void block_gram(const double (*Ps)[NT + 1], double *Gp) {
 // so that the 16 lanes of a group hold 16 consecutive rows of one column and
 // every store to the column-major outputs covers 128 contiguous bytes.
 // Results go to out (global, ld, rows row0.. below m), to out2 as well if
-// non-null and, if Os is non-null, to Os[col][row] in LDS.
+// non-null and, if Os is non-null, to Os[col][row] in LDS, after a barrier
+// (so Os may be Ls).
 __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, double *out, long ld,
                                int row0, int m, double (*Os)[NT + 1], double *out2 = nullptr) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 15, lk = lane >> 4;
@@ -640,8 +641,16 @@ __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, doub
         const int rl = 64 * wv + 16 * rt + lr, c = 16 * ct + lk + 4 * v;
         if (row0 + rl < m) out[row0 + rl + (long)c * ld] = acc[rt][ct][v];
         if (out2 && row0 + rl < m) out2[row0 + rl + (long)c * ld] = acc[rt][ct][v];
-        if (Os) Os[c][rl] = acc[rt][ct][v];
       }
+  if (Os) {
+    __syncthreads();  // every wave has read Ls
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Os[16 * ct + lk + 4 * v][64 * wv + 16 * rt + lr] = acc[rt][ct][v];
+  }
 }
 
 // Last of the grid's blocks to arrive sums the partial Grams into G (LDS),
@@ -723,9 +732,10 @@ __global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__re
                                                 double *__restrict__ Gp, int *cnt,
                                                 double *__restrict__ Rbuf, int *fail) {
   if (*fail) return;
+  // 91 KB of LDS (Q1 replaces P in place), so that a workgroup fits on a CU
+  // beside one of the look-ahead SYR2K
   __shared__ double Ps[B][NT + 1];
   __shared__ Mat32 R[B], Ri[B], G[B];
-  __shared__ double Qs[B][NT + 1];
   const int t = threadIdx.x, row = NT * blockIdx.x + t;
   {
     double v[B], rv[B * B / NT];
@@ -738,11 +748,10 @@ __global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__re
     for (int k = 0; k < B; ++k) Ps[k][t] = v[k];
   }
   __syncthreads();
-  // Q1 = P R1^{-1} for the block's rows, to global and to LDS for the Gram
-  rows_mm(Ps, Ri, Q1, ldq, NT * blockIdx.x, m, Qs);
+  // Q1 = P R1^{-1} for the block's rows, to global and into Ps for the Gram
+  rows_mm(Ps, Ri, Q1, ldq, NT * blockIdx.x, m, Ps);
   __syncthreads();
-  const double (*Gs)[NT + 1] = Qs;
-  block_gram(Gs, Gp + (long)blockIdx.x * B * B);
+  block_gram(Ps, Gp + (long)blockIdx.x * B * B);
   if (!gram_reduce_last(Gp, cnt, G)) return;
   double dev = 0.0;
   for (int e = t; e < B * B; e += NT)
