@@ -482,6 +482,14 @@ __device__ inline double rcp_fast(double x) {
   return r;
 }
 
+// The value of lane src, wave-uniform.
+__device__ inline double lane_value(double v, int src) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // true in the last of `total` workgroups to call it on cnt
 __device__ inline bool arrive_last(int *cnt, int total, int *last) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -492,104 +500,120 @@ __device__ inline bool arrive_last(int *cnt, int total, int *last) {
   return *last != 0;
 }
 
-// C = alpha A Bm, all 32 x 32 in LDS with explicit zeros (C distinct from A, Bm)
-__device__ inline void lds_mm(Mat32 *C, const Mat32 *A, const Mat32 *Bm, double alpha) {
-  for (int e = threadIdx.x; e < B * B; e += NT) {
-    const int i = e / B, k = e % B;
-    double s = 0.0;
+// C = alpha A op(Bm) for 32 x 32 matrices in LDS on MFMA, op(Bm) = Bm^T if
+// TB: wave v computes the 16 x 16 block (v >> 1, v & 1) of C.  The caller
+// synchronises before C is read.
+template <bool TB = false>
+__device__ inline void mm32(Mat32 *C, const Mat32 *A, const Mat32 *Bm, double alpha) {
+  static_assert(NT == 256, "one 16 x 16 block per wave");
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lr = lane & 15, lk = lane >> 4;
+  const int ti = wv >> 1, tj = wv & 1;
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int l = 0; l < B; ++l) s += A[i][l] * Bm[l][k];
-    C[i][k] = alpha * s;
+  for (int k4 = 0; k4 < B / 4; ++k4) {
+    const double a = A[16 * ti + lr][4 * k4 + lk];
+    const double b = TB ? Bm[16 * tj + lr][4 * k4 + lk] : Bm[4 * k4 + lk][16 * tj + lr];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
   }
-  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < 4; ++v) C[16 * ti + lk + 4 * v][16 * tj + lr] = alpha * acc[v];
 }
 
-// Ai = A^{-1} for A upper triangular (zeros below), by doubling: with the
-// diagonal blocks of size b inverted, each pair of them is completed as
-// [X11 X12; 0 X22] with X12 = -X11 A12 X22, one dot of at most b terms per
-// element and two barriers per level (five levels).  T: scratch.
-__device__ inline void lds_inv_upper(const Mat32 *A, Mat32 *Ai, Mat32 *T) {
-  const int t = threadIdx.x;
-  for (int e = t; e < B * B; e += NT) {
-    const int i = e / B, j = e % B;
-    Ai[i][j] = (i == j) ? 1.0 / A[i][i] : 0.0;
-  }
-  __syncthreads();
+// Upper Cholesky G = R^T R of a 32 x 32 matrix by one wave, without
+// barriers (call from wave 0; G full symmetric in LDS).  Lanes 0..31 hold the
+// columns of G and lanes 32..63 those of the identity; step k applies the
+// same row operations to both (row i -= G[i][k] / G[k][k] row k, i > k),
+// column k read from lane k through the scalar unit.  The identity ends as
+// L^{-1} for G = L D L^T; with D^{-1/2} = diag(rs), R[k][j] = rs[k] G[k][j]
+// (row k as it is when it is the pivot row) and R^{-1}[j][k] =
+// rs[k] L^{-1}[k][j], both final at step k.  Writes R and Ri, zeros outside the triangles; false
+// when a pivot is not above tol (that step divides by 1 and carries on).
+__device__ __forceinline__ bool wave_chol32(const Mat32 *G, Mat32 *R, Mat32 *Ri, double tol) {
+  const int lane = threadIdx.x & 63, j = lane & 31;
+  const bool ih = lane >= B;  // identity half
+  double x[B];
 #pragma unroll
-  for (int b = 1; b < B; b *= 2) {
-    const bool act = t < 16 * b;  // B / (2 b) pairs of b x b blocks
-    const int pr = t / (b * b), rem = t % (b * b);
-    const int o = 2 * b * pr, i = o + rem / b, j = o + b + rem % b;
-    // fixed trip counts with zero factors past the triangles, so that the
-    // LDS reads of a dot issue together
-    if (act) {
-      double s = 0.0;
+  for (int i = 0; i < B; ++i) x[i] = G[i][j];
 #pragma unroll
-      for (int l = 0; l < b; ++l) {
-        const double a = A[i][o + b + l], x = Ai[o + b + l][j];  // A12 X22
-        s += (o + b + l <= j) ? a * x : 0.0;
-      }
-      T[i][j] = s;
-    }
-    __syncthreads();
-    if (act) {
-      double s = 0.0;
-#pragma unroll
-      for (int l = 0; l < b; ++l) {
-        const double x = Ai[i][o + l], y = T[o + l][j];  // X11 (A12 X22)
-        s += (o + l >= i) ? x * y : 0.0;
-      }
-      Ai[i][j] = -s;
-    }
-    __syncthreads();
-  }
-}
-
-// Upper Cholesky G = R^T R: R (zeros below) from the upper triangle of G,
-// which is overwritten, and Ri = R^{-1}.  False when a pivot is not above
-// tol.  Symmetric elimination in place with one barrier per step (step k
-// reads row k, which it does not write); the same row operations on the
-// identity E give E = L^{-1} for G = L D L^T, and R^{-T} = D^{-1/2} E, so
-// row k of E scaled by the pivot's inverse square root is column k of Ri.
-__device__ inline bool lds_chol32(Mat32 *G, Mat32 *R, Mat32 *Ri, Mat32 *E, double tol) {
-  const int t = threadIdx.x;
-  for (int e = t; e < B * B; e += NT) {
-    const int i = e / B, j = e % B;
-    R[i][j] = 0.0;
-    Ri[i][j] = 0.0;
-    E[i][j] = (i == j) ? 1.0 : 0.0;
-  }
-  __syncthreads();
+  for (int i = 0; i < B; ++i) x[i] = ih ? (i == j ? 1.0 : 0.0) : x[i];
   bool ok = true;
-  // thread t: column j = t % B, rows i0 + 8 q
-  constexpr int NQ = B * B / NT;
-  const int j = t % B, i0 = t / B;
+#pragma unroll
   for (int k = 0; k < B; ++k) {
-    // every LDS read of the step first, unconditionally (a read under a
-    // lane condition is waited for before the next one is issued)
-    const double d = G[k][k], gkj = G[k][j], ekj = E[k][j];
-    double gki[NQ], gij[NQ], eij[NQ];
+    const double d = lane_value(x[k], k);
+    ok = ok && d > tol;
+    const double dd = d > tol ? d : 1.0;
+    // row k of both halves is final: R's row k, R^{-1}'s column k
+    const bool in = ih ? (j <= k) : (k <= j);
+    double *p = ih ? &Ri[j][k] : &R[k][j];
+    *p = in ? x[k] * rsq_fast(dd) : 0.0;
+    if (k == B - 1) break;
+    // columns j <= k of G are done; lane k keeps its column as it is
+    const double a = (ih || j > k) ? x[k] * rcp_fast(dd) : 0.0;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int i = i0 + (NT / B) * q;
-      gki[q] = G[k][i];
-      gij[q] = G[i][j];
-      eij[q] = E[i][j];
-    }
-    ok = ok && (d > tol);
-    const double dd = (d > tol) ? d : 1.0;
-    const double rs = rsq_fast(dd), rdd = rs * rs;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int i = i0 + (NT / B) * q;
-      if (i == k && j >= k) R[k][j] = gkj * rs;
-      if (i == k && j <= k) Ri[j][k] = ekj * rs;
-      if (i > k && j >= i) G[i][j] = gij[q] - gki[q] * gkj * rdd;
-      if (i > k && j <= k) E[i][j] = eij[q] - (gki[q] * rdd) * ekj;
-    }
-    __syncthreads();
+    for (int i = k + 1; i < B; ++i) x[i] = fma(-lane_value(x[i], k), a, x[i]);
   }
   return ok;
+}
+
+// Householder reconstruction's LU by one wave, without barriers (call from
+// wave 0): W = Wm S - I = L U without pivoting, S = diag(sgn) with sgn[c] =
+// -sign(Wm[c][c]), so that every pivot has magnitude at least 1.  Lanes
+// 0..31 hold the columns of W and lanes 32..63 those of the identity, which
+// the same row operations turn into L^{-1}; pivot column k is read from lane
+// k through the scalar unit.  Then lane c solves U u = e_c from the bottom up
+// with U's columns read from lanes 0..31.  Writes sgn, Lu = L (unit lower,
+// zeros above), Ui = S U^{-1}, Z = S L^{-T}, and Wm - S over Wm.
+__device__ __forceinline__ void wave_lu32(Mat32 *Wm, double *sgn, Mat32 *Lu, Mat32 *Ui, Mat32 *Z) {
+  const int lane = threadIdx.x & 63, c = lane & 31;
+  const bool ih = lane >= B;  // identity half
+  double x[B], rp[B];
+#pragma unroll
+  for (int r = 0; r < B; ++r) x[r] = Wm[r][c];
+  double dcc = 0.0;
+#pragma unroll
+  for (int r = 0; r < B; ++r) dcc = (r == c) ? x[r] : dcc;
+  const double sc = dcc >= 0.0 ? -1.0 : 1.0;
+  if (!ih) {
+    sgn[c] = sc;
+    Wm[c][c] = dcc - sc;
+  }
+#pragma unroll
+  for (int r = 0; r < B; ++r) x[r] = ih ? (r == c ? 1.0 : 0.0) : x[r] * sc - (r == c ? 1.0 : 0.0);
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const double piv = lane_value(x[k], k);
+    rp[k] = rcp_fast(piv);  // |piv| >= 1
+    if (k == B - 1) break;
+    // lanes c <= k keep their columns: U above the diagonal, L's multipliers
+    // (times the pivot) below it
+    const double a = (ih || c > k) ? x[k] * rp[k] : 0.0;
+#pragma unroll
+    for (int r = k + 1; r < B; ++r) x[r] = fma(-lane_value(x[r], k), a, x[r]);
+  }
+  double u[B];
+#pragma unroll
+  for (int i = 0; i < B; ++i) u[i] = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+  for (int l = B - 1; l >= 0; --l) {
+    // U's columns do not change here: without the barrier the scheduler
+    // hoists every step's lane reads and spills them
+    __builtin_amdgcn_sched_barrier(0);
+    u[l] *= rp[l];
+#pragma unroll
+    for (int i = 0; i < l; ++i) u[i] = fma(-lane_value(x[i], l), u[l], u[i]);
+  }
+  // no branch on the half: every lane's u is the same as its partner's, and
+  // lane reads under a divergent branch would be hoisted and spilled
+  double rpc = 0.0;
+#pragma unroll
+  for (int k = 0; k < B; ++k) rpc = (c == k) ? rp[k] : rpc;
+#pragma unroll
+  for (int r = 0; r < B; ++r) {
+    // identity half: x = column c of L^{-1}, row c of Z
+    double *p = ih ? &Z[c][r] : &Lu[r][c];
+    *p = ih ? sc * x[r] : ((r > c) ? x[r] * rpc : (r == c ? 1.0 : 0.0));
+    Ui[r][c] = u[r] * lane_value(sc, r);
+  }
 }
 
 // Partial Gram Ps^T Ps of this block's rows (Ps[col][row], NT rows) on MFMA:
@@ -615,9 +639,11 @@ __device__ inline void block_gram(const double (*Ps)[NT + 1], double *Gp) {
 // every store to the column-major outputs covers 128 contiguous bytes.
 // Results go to out (global, ld, rows row0.. below m), to out2 as well if
 // non-null and, if Os is non-null, to Os[col][row] in LDS, after a barrier
-// (so Os may be Ls).
+// (so Os may be Ls).  With top non-null (row0 = 0), rows 0 .. B-1 of the
+// results are top's instead.
 __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, double *out, long ld,
-                               int row0, int m, double (*Os)[NT + 1], double *out2 = nullptr) {
+                               int row0, int m, double (*Os)[NT + 1], double *out2 = nullptr,
+                               const Mat32 *top = nullptr) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 15, lk = lane >> 4;
   v4d acc[4][2];
 #pragma unroll
@@ -631,6 +657,14 @@ __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, doub
       acc[rt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a, acc[rt][0], 0, 0, 0);
       acc[rt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a, acc[rt][1], 0, 0, 0);
     }
+  }
+  if (top && wv == 0) {
+#pragma unroll
+    for (int rt = 0; rt < B / 16; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[rt][ct][v] = top[16 * rt + lr][16 * ct + lk + 4 * v];
   }
 #pragma unroll
   for (int rt = 0; rt < 4; ++rt)
@@ -713,10 +747,15 @@ __global__ __launch_bounds__(NT) void k_sbr_cq1(int m, int w, const double *__re
   __syncthreads();
   block_gram(Ps, Gp + (long)blockIdx.x * B * B);
   if (!gram_reduce_last(Gp, cnt, G)) return;
+  __shared__ int okf;
   double dmax = 0.0;
   for (int k = 0; k < B; ++k) dmax = fmax(dmax, G[k][k]);
-  Mat32 *const E = reinterpret_cast<Mat32 *>(&Ps[0][0]);  // the panel rows are no longer needed
-  const bool ok = lds_chol32(G, R, Ri, E, 1e-10 * dmax) && dmax > 0.0;
+  if (t < 64) {
+    const bool o = wave_chol32(G, R, Ri, 1e-10 * dmax);
+    if (t == 0) okf = o;
+  }
+  __syncthreads();
+  const bool ok = okf && dmax > 0.0;
   for (int e = t; e < B * B; e += NT) {
     Rbuf[e] = R[e / B][e % B];
     Rbuf[2 * B * B + e] = Ri[e / B][e % B];
@@ -757,8 +796,13 @@ __global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__re
   for (int e = t; e < B * B; e += NT)
     dev = fmax(dev, fabs(G[e / B][e % B] - (e / B == e % B ? 1.0 : 0.0)));
   dev = block_max(dev);
-  Mat32 *const E = reinterpret_cast<Mat32 *>(&Ps[0][0]);  // the panel rows are no longer needed
-  const bool ok = (dev < 0.1) && lds_chol32(G, R, Ri, E, 0.5);
+  __shared__ int okf;
+  if (t < 64) {
+    const bool o = wave_chol32(G, R, Ri, 0.5);
+    if (t == 0) okf = o;
+  }
+  __syncthreads();
+  const bool ok = (dev < 0.1) && okf;
   for (int e = t; e < B * B; e += NT) {
     Rbuf[B * B + e] = R[e / B][e % B];
     Rbuf[3 * B * B + e] = Ri[e / B][e % B];
@@ -766,149 +810,69 @@ __global__ __launch_bounds__(NT) void k_sbr_cq2(int m, int w, const double *__re
   if (t == 0) *fail = ok ? 0 : 1;
 }
 
-// Q = Q1 R2^{-1}, Householder reconstruction of Q as in k_sbr_qform, every
-// per-row step folded into two B x B matrices: y = q1 My, x = y T = q1 Mx with
-// My = R2^{-1} S U^{-1}, Mx = My T (rows of the top block: y = Y1).  Writes V
-// (into V1 and V2), X = V T and, from block 0, S R2 R1.
+// Q = Q1 R2^{-1}, Householder reconstruction of Q as in k_sbr_qform: with
+// W = Q_top S - I = L U (the top B x B block, S = -sign diag(Q_top)), the
+// reflectors are V = [L; Q_rest S U^{-1}] and X = V T with T = -U L^{-T}, so
+// every row below the top block is one product with a B x B matrix:
+// v = q1 My, x = q1 Mx, My = R2^{-1} S U^{-1}, Mx = -R2^{-1} S L^{-T}; the
+// top block's rows are L and -W L^{-T} = -(Q1_top R2^{-1} - S) S L^{-T}.
+// Writes V (into V1 and V2), X and, from block 0, the band block S R2 R1.
 __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict__ Q1, long ldq,
                                                 const double *__restrict__ Rbuf, const int *fail,
                                                 double *__restrict__ V1, double *__restrict__ V2,
                                                 long ldv, double *__restrict__ X, long ldx,
                                                 double *__restrict__ Ab, long lda) {
   if (*fail) return;
-  __shared__ Mat32 R2[B], R2i[B], Wm[B], U[B], Y1[B], Ui[B], Yi[B], Tf[B], My[B], Mx[B];
+  __shared__ Mat32 R2i[B], Wm[B], Lu[B], Ui[B], Z[B], My[B], Mx[B], Xt[B];
   __shared__ double Qb[B][NT + 1];  // the block's rows of Q1
   __shared__ double sgn[B];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, g = NT * blockIdx.x + t;
+  const bool top = blockIdx.x == 0;
   {
-    double rv[B * B / NT], riv[B * B / NT], yv[B * B / NT];
+    // every load up front: R2^{-1}, the top block of Q1, the block's rows of
+    // Q1 and, in block 0, R2 and R1 (into Lu and Ui until the LU needs them)
+    double rv[B * B / NT], yv[B * B / NT], r2[B * B / NT], r1[B * B / NT], v[B];
 #pragma unroll
     for (int q = 0; q < B * B / NT; ++q) {
       const int e = t + NT * q;
-      rv[q] = Rbuf[B * B + e];
-      riv[q] = Rbuf[3 * B * B + e];
-      yv[q] = Q1[e / B + (long)(e % B) * ldq];  // top block of Q1
+      rv[q] = Rbuf[3 * B * B + e];
+      yv[q] = Q1[e / B + (long)(e % B) * ldq];
+      r2[q] = top ? Rbuf[B * B + e] : 0.0;
+      r1[q] = top ? Rbuf[e] : 0.0;
     }
-#pragma unroll
-    for (int q = 0; q < B * B / NT; ++q) {
-      const int e = t + NT * q;
-      R2[e / B][e % B] = rv[q];
-      R2i[e / B][e % B] = riv[q];
-      Y1[e / B][e % B] = yv[q];
-    }
-  }
-  __syncthreads();
-  lds_mm(Wm, Y1, R2i, 1.0);  // top block of Q
-  if (t < B) sgn[t] = (Wm[t][t] >= 0.0) ? -1.0 : 1.0;
-  __syncthreads();
-  for (int e = t; e < B * B; e += NT) {
-    const int r = e / B, k = e % B;
-    Wm[r][k] = Wm[r][k] * sgn[k] - (r == k ? 1.0 : 0.0);
-  }
-  __syncthreads();
-  // LU without pivoting (|pivot| >= 1 by the choice of signs) and Yi = L^{-1}
-  // by the same row operations on the identity, one barrier per step: step
-  // kk reads row kk and column kk and writes neither; the multipliers of
-  // column kk go in at step kk + 1, after every thread has read the column.
-  for (int e = t; e < B * B; e += NT) Yi[e / B][e % B] = (e / B == e % B) ? 1.0 : 0.0;
-  __syncthreads();
-  {
-    double lprev[B * B / NT];
-#pragma unroll
-    for (int q = 0; q < B * B / NT; ++q) lprev[q] = 0.0;
-    constexpr int NQ = B * B / NT;
-    const int c = t % B, r0 = t / B;  // thread t: column c, rows r0 + 8 q
-    for (int kk = 0; kk < B - 1; ++kk) {
-      // every LDS read of the step first, unconditionally
-      const double piv = Wm[kk][kk], wkc = Wm[kk][c], ykc = Yi[kk][c];
-      double wrk[NQ], wrc[NQ], yrc[NQ];
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int r = r0 + (NT / B) * q;
-        wrk[q] = Wm[r][kk];
-        wrc[q] = Wm[r][c];
-        yrc[q] = Yi[r][c];
-      }
-      const double rpiv = rcp_fast(piv);  // |piv| >= 1
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int r = r0 + (NT / B) * q;
-        if (kk > 0 && r > kk - 1 && c == kk - 1) Wm[r][c] = lprev[q];
-        const double lm = wrk[q] * rpiv;
-        if (r > kk && c > kk) Wm[r][c] = wrc[q] - lm * wkc;
-        if (r > kk && c <= kk) Yi[r][c] = yrc[q] - lm * ykc;
-        lprev[q] = lm;
-      }
-      __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int r = r0 + (NT / B) * q;
-      if (r > B - 2 && c == B - 2) Wm[r][c] = lprev[q];
-    }
-  }
-  __syncthreads();
-  for (int e = t; e < B * B; e += NT) {
-    const int r = e / B, k = e % B;
-    U[r][k] = (k >= r) ? Wm[r][k] : 0.0;
-    Y1[r][k] = (k < r) ? Wm[r][k] : 0.0;
-  }
-  __syncthreads();
-  lds_inv_upper(U, Ui, Tf);
-  // Tf = -U Y1^{-T}
-  for (int e = t; e < B * B; e += NT) {
-    const int i = e / B, k = e % B;
-    double s = 0.0;
-#pragma unroll
-    for (int l = 0; l < B; ++l) s += U[i][l] * Yi[k][l];
-    Tf[i][k] = -s;
-  }
-  // My = R2^{-1} S U^{-1}: scale the rows of U^{-1} by S in place first
-  for (int e = t; e < B * B; e += NT) Ui[e / B][e % B] *= sgn[e / B];
-  __syncthreads();
-  lds_mm(My, R2i, Ui, 1.0);
-  lds_mm(Mx, My, Tf, 1.0);
-  const int g = NT * blockIdx.x + t;
-  // y = q1 My (into V1 and V2), x = q1 Mx, for every row on MFMA; the top
-  // block's rows are then replaced by their exact form below
-  {
-    double v[B];
     load_row(Q1, ldq, g, m, B, v);
+#pragma unroll
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q;
+      R2i[e / B][e % B] = rv[q];
+      Xt[e / B][e % B] = yv[q];
+      Lu[e / B][e % B] = r2[q];
+      Ui[e / B][e % B] = r1[q];
+    }
 #pragma unroll
     for (int k = 0; k < B; ++k) Qb[k][t] = v[k];
   }
   __syncthreads();
-  rows_mm(Qb, My, V1, ldv, NT * blockIdx.x, m, nullptr, V2);
-  rows_mm(Qb, Mx, X, ldx, NT * blockIdx.x, m, nullptr);
-  __syncthreads();  // the product's stores to the top rows land before their replacement
-  if (g < m && g < B) {
-    double qr[B];
-    {
+  mm32(Wm, Xt, R2i, 1.0);  // Q_top S = Q1_top R2^{-1} S
+  if (top) mm32(My, Lu, Ui, 1.0);  // R2 R1
+  __syncthreads();
+  if (t < 64) wave_lu32(Wm, sgn, Lu, Ui, Z);
+  __syncthreads();
+  if (top) {
+    // band block S R, R = R2 R1 (in My), rows down the lanes
 #pragma unroll
-      for (int k = 0; k < B; ++k) qr[k] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
-      for (int k = 0; k < B; ++k) {
-        double x = 0.0;
-#pragma unroll
-        for (int i = 0; i < B; ++i) x += qr[i] * Tf[i][k];
-        V1[g + (long)k * ldv] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
-        V2[g + (long)k * ldv] = (k < g) ? Y1[g][k] : (k == g ? 1.0 : 0.0);
-        X[g + (long)k * ldx] = x;
-      }
+    for (int q = 0; q < B * B / NT; ++q) {
+      const int e = t + NT * q, i = e % B, k = e / B;
+      Ab[i + (long)k * lda] = (k >= i) ? sgn[i] * My[i][k] : 0.0;
     }
-  }
-  if (blockIdx.x == 0) {
-    // band block S R, R = R2 R1 (Wm is free again)
     __syncthreads();
-    double rv[B * B / NT];
-#pragma unroll
-    for (int q = 0; q < B * B / NT; ++q) rv[q] = Rbuf[t + NT * q];
-#pragma unroll
-    for (int q = 0; q < B * B / NT; ++q) Y1[(t + NT * q) / B][(t + NT * q) % B] = rv[q];
-    __syncthreads();
-    lds_mm(Wm, R2, Y1, 1.0);
-    if (t < B)
-      for (int k = 0; k < B; ++k) Ab[t + (long)k * lda] = (k >= t) ? sgn[t] * Wm[t][k] : 0.0;
   }
+  mm32(My, R2i, Ui, 1.0);
+  mm32(Mx, R2i, Z, -1.0);
+  mm32(Xt, Wm, Z, -1.0);
+  __syncthreads();
+  rows_mm(Qb, My, V1, ldv, NT * blockIdx.x, m, nullptr, V2, top ? Lu : nullptr);
+  rows_mm(Qb, Mx, X, ldx, NT * blockIdx.x, m, nullptr, nullptr, top ? Xt : nullptr);
 }
 
 namespace {
@@ -1403,14 +1367,6 @@ __device__ inline double xor32(double v) {
   const bool up = (threadIdx.x & 32) != 0;
   const unsigned rlo = up ? a[0] : a[1], rhi = up ? b[0] : b[1];
   return __longlong_as_double((long long)(((unsigned long long)rhi << 32) | rlo));
-}
-
-// The value of lane src, wave-uniform.
-__device__ inline double lane_value(double v, int src) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, src);
-  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // 16-byte accesses to the band with the same cache policy as the agent-scope
