@@ -2,15 +2,18 @@
 // (the reduction inside PDSYEV('N','L') at UTILS.f90:835-858, eigenvalues only).
 //
 // Stage 1, dense -> band of half-width B = 32, one panel of B columns at a time:
-//   k_sbr_tsqr   QR of the m x B panel below the diagonal block as a tree
-//                (TSQR): each 256-row leaf is factored by one workgroup with
-//                its rows in registers, the R factors of 8 siblings are
+//   k_sbr_cq1..3 CholeskyQR2 of the m x B panel below the diagonal block (Gram
+//                passes on MFMA summed by the last workgroup to arrive, the
+//                32 x 32 Cholesky factorisations by one wave), then
+//                Householder reconstruction (LU without pivoting of Q S -
+//                [I; 0], S = -sign diag(Q), also by one wave) into compact-WY
+//                reflectors V, T with H = I - V T V^T whose first B columns
+//                are Q S; writes V, X = V T and the band block S R;
+//   k_sbr_tsqr   for a panel CholeskyQR2 cannot factor: QR of the panel as a
+//                tree (TSQR): each 256-row leaf is factored by one workgroup
+//                with its rows in registers, the R factors of 8 siblings are
 //                stacked and factored by the last sibling to finish;
-//   k_sbr_qform  explicit Q from the tree, then Householder reconstruction
-//                (LU without pivoting of Q S - [I; 0], S = -sign diag(Q))
-//                gives compact-WY reflectors V, T with H = I - V T V^T whose
-//                first B columns are Q S; writes V, X = V T and the band
-//                block S R;
+//   k_sbr_qform  explicit Q from the tree and the same reconstruction;
 //   k_sbr_symm   Y = A22 X from the lower triangle on MFMA (FP64 16x16x4),
 //                the columns split over workgroups into partials;
 //   k_sbr_zred   sums the partials, M = X^T Y by a two-level arrival tree;
@@ -21,8 +24,9 @@
 // chases the bulge down in tasks of B rows.  A workgroup runs three
 // consecutive sweeps, one wavefront each, in lock step over a window of band
 // columns in LDS, with a fourth wavefront loading the columns the group's
-// first sweep needs from the previous group; groups pipeline across a
-// persistent grid through progress counters and mailboxes in memory.
+// first sweep needs from the previous group and a fifth storing what its last
+// sweep leaves behind; groups pipeline across a persistent grid through
+// progress counters and mailboxes in memory.
 #include <algorithm>
 #include <cstdlib>
 

@@ -39,7 +39,7 @@ def test_more_than_one_gpu_exits_with_message():
 def test_traffic_ratio_read_from_profiles():
     sys.path.insert(0, ROOT)
     import bench
-    assert abs(bench.measured_traffic_ratio("sbr_chase") - 0.3313) < 1e-12
+    assert abs(bench.measured_traffic_ratio("sbr_chase") - 0.3310) < 1e-12
     assert abs(bench.measured_traffic_ratio("sbr_symm") - 2.1700) < 1e-12
     assert abs(bench.measured_traffic_ratio("pcg_symv") - 1.1017) < 1e-12
     assert abs(bench.measured_traffic_ratio("scs_build") - 1.0196) < 1e-12
