@@ -202,58 +202,65 @@ __global__ __launch_bounds__(256) void k_gemm_ts(int m, int n, int k, double alp
   v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
   const int nch = (k + TS_KC - 1) / TS_KC, nterm = A2 ? 2 : 1, nstep = nterm * nch;
   // A chunk: element e = t + 256 i is row e % 64, column w + 4 i; B chunk:
-  // row e % 32, column e / 32; clamped addresses, zeros past m, n and k
-  double ra[TS_M * TS_KC / 256], rb[TS_N * TS_KC / 256];
-  auto gload = [&](int st) {
+  // row e % 32, column e / 32; clamped addresses, zeros past m, n and k.
+  // Up to TS_G chunks (every chunk of the panel updates, K <= 128) are loaded
+  // at once, and C with them, so that one memory latency covers the kernel.
+  constexpr int TS_G = 4;
+  double ra[TS_G][TS_M * TS_KC / 256], rb[TS_G][TS_N * TS_KC / 256];
+  auto gload = [&](int st, int j) {
     const int term = st / nch, k0 = (st % nch) * TS_KC;
     const double *A = term ? A2 : A1, *Bm = term ? B2 : B1;
 #pragma unroll
     for (int i = 0; i < TS_M * TS_KC / 256; ++i) {
       const int kk = k0 + w + 4 * i;
       const double v = A[rr + (long)min(kk, k - 1) * lda];
-      ra[i] = (kk < k && row0 + r < m) ? v : 0.0;
+      ra[j][i] = (kk < k && row0 + r < m) ? v : 0.0;
     }
 #pragma unroll
     for (int i = 0; i < TS_N * TS_KC / 256; ++i) {
       const int e = t + 256 * i, c = e & (TS_N - 1), kk = k0 + (e >> 5);
       const double v = Bm[min(c, n - 1) + (long)min(kk, k - 1) * ldb];
-      rb[i] = (c < n && kk < k) ? v : 0.0;
+      rb[j][i] = (c < n && kk < k) ? v : 0.0;
     }
   };
-  gload(0);
-  for (int st = 0; st < nstep; ++st) {
-    __syncthreads();  // the previous chunk's LDS reads are done
+  // transposed blocks: lane (lk, lr) holds C[row 16 w + lr][col lk + 4 v] (+16)
+  const int row = row0 + 16 * w + lr, crow = min(row, m - 1);
+  double cv[8];
 #pragma unroll
-    for (int i = 0; i < TS_M * TS_KC / 256; ++i) As[w + 4 * i][r] = ra[i];
+  for (int v = 0; v < 4; ++v) {
+    cv[v] = beta != 0.0 ? C[crow + (long)min(lk + 4 * v, n - 1) * ldc] : 0.0;
+    cv[4 + v] = beta != 0.0 ? C[crow + (long)min(16 + lk + 4 * v, n - 1) * ldc] : 0.0;
+  }
+  for (int g0 = 0; g0 < nstep; g0 += TS_G) {
 #pragma unroll
-    for (int i = 0; i < TS_N * TS_KC / 256; ++i) {
-      const int e = t + 256 * i;
-      Bs[e >> 5][e & (TS_N - 1)] = rb[i];
-    }
-    __syncthreads();
-    if (st + 1 < nstep) gload(st + 1);
+    for (int j = 0; j < TS_G; ++j) gload(min(g0 + j, nstep - 1), j);
 #pragma unroll
-    for (int q = 0; q < TS_KC / 4; ++q) {
-      const double a = As[4 * q + lk][16 * w + lr];
-      const double b0 = Bs[4 * q + lk][lr], b1 = Bs[4 * q + lk][16 + lr];
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a, acc1, 0, 0, 0);
+    for (int j = 0; j < TS_G; ++j) {
+      if (g0 + j >= nstep) break;
+      __syncthreads();  // the previous chunk's LDS reads are done
+#pragma unroll
+      for (int i = 0; i < TS_M * TS_KC / 256; ++i) As[w + 4 * i][r] = ra[j][i];
+#pragma unroll
+      for (int i = 0; i < TS_N * TS_KC / 256; ++i) {
+        const int e = t + 256 * i;
+        Bs[e >> 5][e & (TS_N - 1)] = rb[j][i];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < TS_KC / 4; ++q) {
+        const double a = As[4 * q + lk][16 * w + lr];
+        const double b0 = Bs[4 * q + lk][lr], b1 = Bs[4 * q + lk][16 + lr];
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a, acc1, 0, 0, 0);
+      }
     }
   }
-  // transposed blocks: lane (lk, lr) holds C[row 16 w + lr][col lk + 4 v] (+16)
-  const int row = row0 + 16 * w + lr;
   if (row < m) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int c0 = lk + 4 * v, c1 = 16 + lk + 4 * v;
-      if (c0 < n) {
-        double *pc = C + row + (long)c0 * ldc;
-        *pc = alpha * acc0[v] + (beta != 0.0 ? beta * *pc : 0.0);
-      }
-      if (c1 < n) {
-        double *pc = C + row + (long)c1 * ldc;
-        *pc = alpha * acc1[v] + (beta != 0.0 ? beta * *pc : 0.0);
-      }
+      if (c0 < n) C[row + (long)c0 * ldc] = alpha * acc0[v] + beta * cv[v];
+      if (c1 < n) C[row + (long)c1 * ldc] = alpha * acc1[v] + beta * cv[4 + v];
     }
   }
 }

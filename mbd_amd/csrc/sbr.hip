@@ -692,7 +692,7 @@ __device__ inline void rows_mm(const double (*Ls)[NT + 1], const Mat32 *Mt, doub
 }
 
 // Last of the grid's blocks to arrive sums the partial Grams into G (LDS),
-// 16 loads in flight per element.
+// 16 loads in flight per element, every element's together.
 __device__ inline bool gram_reduce_last(const double *Gp, int *cnt, Mat32 *G) {
   __shared__ int last;
   if (!arrive_last(cnt, (int)gridDim.x, &last)) return false;
@@ -702,15 +702,18 @@ __device__ inline bool gram_reduce_last(const double *Gp, int *cnt, Mat32 *G) {
 #pragma unroll
   for (int q = 0; q < B * B / NT; ++q) acc[q] = 0.0;
   for (int b0 = 0; b0 < nb; b0 += 16) {
+    // the loads of all B * B / NT elements together (atomic loads are not
+    // moved across each other)
+    double v[B * B / NT][16];
 #pragma unroll
-    for (int q = 0; q < B * B / NT; ++q) {
-      const int e = threadIdx.x + NT * q;
-      double v[16];
+    for (int q = 0; q < B * B / NT; ++q)
 #pragma unroll
-      for (int i = 0; i < 16; ++i) v[i] = ld_coh(&Gp[(long)min(b0 + i, nb - 1) * B * B + e]);
+      for (int i = 0; i < 16; ++i)
+        v[q][i] = ld_coh(&Gp[(long)min(b0 + i, nb - 1) * B * B + threadIdx.x + NT * q]);
 #pragma unroll
-      for (int i = 0; i < 16; ++i) acc[q] += (b0 + i < nb) ? v[i] : 0.0;
-    }
+    for (int q = 0; q < B * B / NT; ++q)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[q] += (b0 + i < nb) ? v[q][i] : 0.0;
   }
 #pragma unroll
   for (int q = 0; q < B * B / NT; ++q) {
@@ -1041,23 +1044,25 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
   if (!arrive_last(&mcnt[1 + grp], gsz, &last)) return;
   if (t == 0) mcnt[1 + grp] = 0;
   double *Mg = Mp + (long)nb * NQ;
-  {
-    double acc[NQ / 256];
+  // four elements per thread per round (nq / 256 is 4 or 12), their 4 x G
+  // loads issued together
+  constexpr int U = 4;
 #pragma unroll
-    for (int i = 0; i < NQ / 256; ++i) {
-      const int e = t + 256 * i;
-      const int ee = min(e, nq - 1);
-      double v[G];
+  for (int r = 0; r < NQ / (256 * U); ++r) {
+    if (U * r >= nq / 256) break;
+    double v[U][G];
 #pragma unroll
-      for (int q = 0; q < G; ++q) v[q] = ld_coh(&Mp[(long)(grp * G + min(q, gsz - 1)) * NQ + ee]);
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < G; ++q)
+        v[u][q] = ld_coh(&Mp[(long)(grp * G + min(q, gsz - 1)) * NQ + t + 256 * (U * r + u)]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
       double s = 0.0;
 #pragma unroll
-      for (int q = 0; q < G; ++q) s += (q < gsz) ? v[q] : 0.0;
-      acc[i] = (e < nq) ? s : 0.0;
+      for (int q = 0; q < G; ++q) s += (q < gsz) ? v[u][q] : 0.0;
+      st_coh(&Mg[(long)grp * NQ + t + 256 * (U * r + u)], s);
     }
-#pragma unroll
-    for (int i = 0; i < NQ / 256; ++i)
-      if (t + 256 * i < nq) st_coh(&Mg[(long)grp * NQ + t + 256 * i], acc[i]);
   }
   if (!arrive_last(&mcnt[0], ng, &last)) return;
   if (t == 0) mcnt[0] = 0;
@@ -1065,18 +1070,21 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
   double *gv = &Ys[0][0], *gz = &Xs[0][0];
   double acc[NQ / 256];
 #pragma unroll
-  for (int i = 0; i < NQ / 256; ++i) {
-    const int e = t + 256 * i;
-    const int ee = min(e, nq - 1);
-    double s = 0.0;
-    for (int g0 = 0; g0 < ng; g0 += 8) {
-      double v[8];
+  for (int i = 0; i < NQ / 256; ++i) acc[i] = 0.0;
 #pragma unroll
-      for (int g = 0; g < 8; ++g) v[g] = ld_coh(&Mg[(long)min(g0 + g, ng - 1) * NQ + ee]);
+  for (int r = 0; r < NQ / (256 * U); ++r) {
+    if (U * r >= nq / 256) break;
+    for (int g0 = 0; g0 < ng; g0 += G) {
+      double v[U][G];
 #pragma unroll
-      for (int g = 0; g < 8; ++g) s += (g0 + g < ng) ? v[g] : 0.0;
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int g = 0; g < G; ++g) v[u][g] = ld_coh(&Mg[(long)min(g0 + g, ng - 1) * NQ + t + 256 * (U * r + u)]);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int g = 0; g < G; ++g) acc[U * r + u] += (g0 + g < ng) ? v[u][g] : 0.0;
     }
-    acc[i] = (e < nq) ? s : 0.0;
   }
 #pragma unroll
   for (int i = 0; i < NQ / 256; ++i) {
