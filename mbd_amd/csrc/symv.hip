@@ -54,7 +54,7 @@ Symv3Sizes symv3_sizes(int n) {
 
 // Strip/segment workgroups; blockIdx.y = matrix.
 template <typename T>
-__global__ __launch_bounds__(W) void k_symv3_strip(int n, const T *__restrict__ A, int lda,
+__global__ __launch_bounds__(W, sizeof(T) == 4 ? 3 : 1) void k_symv3_strip(int n, const T *__restrict__ A, int lda,
                                                    long sA, const double *__restrict__ P, int ldp,
                                                    long sP, double *__restrict__ U,
                                                    double *__restrict__ V, long sU, long sV,
@@ -82,44 +82,57 @@ __global__ __launch_bounds__(W) void k_symv3_strip(int n, const T *__restrict__ 
     const int c = e % W, k = e / W;
     pj[k][c] = (J0 + c < n) ? Pb[J0 + c + (long)k * ldp] : 0.0;
   }
-  const int r = t % H, cg = t / H;  // tile load: row r, columns cg + CG k
-  constexpr int KT = W / CG;          // elements per thread per tile (32)
+  // tile load: row r, columns cbase + 2 k (cg = t / H: two column groups per
+  // wave, interleaved, so that a wave's k-th load is one uniform offset away)
+  const int r = t % H, cg = t / H;
+  const int cbase = 64 * (cg >> 1) + (cg & 1);
+  constexpr int KT = W / CG;  // elements per thread per tile (32)
+  static_assert(W == 256 && H == 32 && CG == 8, "two column groups of 32 rows per wave");
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;  // column J0 + t, strictly lower rows
   const int rbeg = J0 + g * H * SEG;
   const int rend = min(n, rbeg + H * SEG);
   // the tile's elements are loaded one tile ahead of their use, so that a
   // thread always has a whole tile of loads in flight
-  T nxt[KT];
-  // clamped addresses and a select instead of a lane condition on the load:
-  // the tile's loads then issue without a branch each (the upper part of a
-  // diagonal tile is read and discarded)
-  auto load_tile = [&](int R0) {
-    const int row = R0 + r;
-    const T *arow = M + min(row, n - 1) + (long)J0 * lda;
+  // Buffer loads from the tile's corner (a wave-uniform resource rebuilt per
+  // tile): one per-lane offset and a uniform offset per column pair, instead
+  // of a 64-bit address per column, and no branch per load.  The resource
+  // ends at the matrix's last element, so the loads past it (rows or columns
+  // beyond n, masked below) read zeros without touching memory; the upper
+  // part of a diagonal tile is read and discarded.
+  const int voff = (r + cbase * lda) * (int)sizeof(T);
+  // Raw values only: the mask is applied where they are used (a select right
+  // after a load makes the compiler wait for that load there).  P's rows of
+  // the tile come with them, loaded by every thread at clamped addresses.
+  auto load_tile = [&](T (&dst)[KT], double &pv, int R0) {
+    const long rest = ((long)(n - 1 - R0) + (long)(n - 1 - J0) * lda + 1) * (long)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<T *>(M + R0 + (long)J0 * lda), (short)0, (int)min(rest, 0x7fffffffL), 0x00020000);
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
-      const int c = cg + CG * k, col = J0 + c;
-      const T v = arow[(long)min(c, n - 1 - J0) * lda];
-      nxt[k] = (row < n && col < n && row >= col) ? v : (T)0;
+      const int soff = 2 * k * lda * (int)sizeof(T);
+      T v;
+      if constexpr (sizeof(T) == 4) {
+        v = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+      } else {
+        const auto u = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0);
+        v = __longlong_as_double((long long)(((unsigned long long)u[1] << 32) | u[0]));
+      }
+      dst[k] = v;
     }
+    pv = Pb[min(R0 + t % H, n - 1) + (long)min(t / H, 2) * ldp];
   };
-  if (rbeg < rend) load_tile(rbeg);
-  for (int R0 = rbeg; R0 < rend; R0 += H) {
-    T cur[KT];
-#pragma unroll
-    for (int k = 0; k < KT; ++k) cur[k] = nxt[k];
-    if (R0 + H < rend) load_tile(R0 + H);
+  auto process = [&](const T (&cur)[KT], double pv, int R0) {
     __syncthreads();
     if (t < 3 * H) {
       const int rr = t % H, k = t / H;
-      pi[k][rr] = (R0 + rr < n) ? Pb[R0 + rr + (long)k * ldp] : 0.0;
+      pi[k][rr] = (R0 + rr < n) ? pv : 0.0;
     }
     const int row = R0 + r;
     double u0 = 0.0, u1 = 0.0, u2 = 0.0;
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
-      const int c = cg + CG * k, col = J0 + c;
-      const T a = cur[k];
+      const int c = cbase + 2 * k, col = J0 + c;
+      const T a = (row < n && col < n && row >= col) ? cur[k] : (T)0;
       const double ad = (double)a;
       u0 += ad * pj[0][c];
       u1 += ad * pj[1][c];
@@ -144,7 +157,26 @@ __global__ __launch_bounds__(W) void k_symv3_strip(int n, const T *__restrict__ 
       v1 += ad * pi[1][rr];
       v2 += ad * pi[2][rr];
     }
+  };
+  // two register sets, each tile loaded two tiles ahead of its use; the
+  // loop body has no branch, so the compiler's wait before a set's use lets
+  // the other set's loads stay in flight (past the end: the last tile again)
+  const int ntl = (rend - rbeg + H - 1) / H;
+  const int rlast = rbeg + H * max(ntl - 1, 0);
+  T ta[KT], tb[KT];
+  double pa = 0.0, pb = 0.0;
+  if (ntl > 0) {
+    load_tile(ta, pa, rbeg);
+    load_tile(tb, pb, min(rbeg + H, rlast));
   }
+  int i = 0;
+  for (; i + 1 < ntl; i += 2) {
+    process(ta, pa, rbeg + H * i);
+    load_tile(ta, pa, min(rbeg + H * (i + 2), rlast));
+    process(tb, pb, rbeg + H * (i + 1));
+    load_tile(tb, pb, min(rbeg + H * (i + 3), rlast));
+  }
+  if (i < ntl) process(ta, pa, rbeg + H * i);
   double *vo = V + (long)b * sV + (((long)J * maxseg + g) * 3) * W;
   vo[t] = v0;
   vo[W + t] = v1;
