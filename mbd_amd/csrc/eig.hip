@@ -580,16 +580,60 @@ __device__ inline int sturm_count(int n, const double *__restrict__ d,
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   cnt += (q < 0.0);
-  for (int i = 1; i < n; ++i) {
+  auto step = [&](double di, double ei) {
     // e2 / q through the hardware reciprocal and one Newton step (within an
-    // ulp or two of the rounded quotient; the recurrence is bound by the
-    // division, whose correctly rounded form takes several times as long)
+    // ulp or two of the rounded quotient; the correctly rounded division
+    // takes several times as long)
     double r = __builtin_amdgcn_rcp(q);
     r = fma(r, fma(-q, r, 1.0), r);
-    q = (d[i] - x) - e2[i - 1] * r;
+    q = (di - x) - ei * r;
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += (q < 0.0);
+  };
+  // Blocks of U steps in two alternating register sets, each block's d and
+  // e2 loaded two blocks ahead, so that their latency is not paid once per
+  // step.  The values are the same in every lane, but they are loaded as
+  // vector loads (through an index with a zero in a vector register): those
+  // complete in order and can be waited for one set at a time, while the
+  // scalar loads the compiler would otherwise use can only be waited for all
+  // at once, the other set's included.
+  constexpr int U = 8;
+  const int nblk = (n - 1) / U;  // full blocks: steps 1 .. U nblk
+  int z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  double da[U], ea[U], db[U], eb[U];
+  auto load = [&](double (&dv)[U], double (&ev)[U], int blk) {
+    const int i0 = 1 + U * min(blk, nblk - 1);  // past the end: the last block again
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      dv[j] = d[i0 + j + z];
+      ev[j] = e2[i0 + j - 1 + z];
+    }
+  };
+  auto run = [&](const double (&dv)[U], const double (&ev)[U]) {
+#pragma unroll
+    for (int j = 0; j < U; ++j) step(dv[j], ev[j]);
+  };
+  if (nblk > 0) {
+    load(da, ea, 0);
+    load(db, eb, 1);
   }
+  int blk = 0;
+  for (; blk + 1 < nblk; blk += 2) {
+    // scheduling barriers: the scheduler would otherwise start a set's
+    // d - x right after its loads, waiting for them a block early
+    run(da, ea);
+    __builtin_amdgcn_sched_barrier(0);
+    load(da, ea, blk + 2);
+    __builtin_amdgcn_sched_barrier(0);
+    run(db, eb);
+    __builtin_amdgcn_sched_barrier(0);
+    load(db, eb, blk + 3);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if (blk < nblk) run(da, ea);
+  int i = 1 + U * nblk;
+  for (; i < n; ++i) step(d[i], e2[i - 1]);
   return cnt;
 }
 
