@@ -64,23 +64,6 @@ struct Sym6 {
   T v[6];
 };
 
-// Neumaier-compensated accumulator for the FP32 lattice sums.
-struct Comp6 {
-  float s[6], c[6];
-  __device__ void zero() {
-    for (int i = 0; i < 6; ++i) s[i] = c[i] = 0.f;
-  }
-  __device__ void add(int i, float x) {
-    float t = s[i] + x;
-    if (fabsf(s[i]) >= fabsf(x))
-      c[i] += (s[i] - t) + x;
-    else
-      c[i] += (x - t) + s[i];
-    s[i] = t;
-  }
-  __device__ float get(int i) const { return s[i] + c[i]; }
-};
-
 // Write a symmetric 3x3 (6 entries) into the LDS tile at block (bi, bj).
 template <typename T>
 __device__ inline void put_block(T (*S)[TBP], int bi, int bj, const T *v6) {
@@ -209,13 +192,26 @@ __global__ __launch_bounds__(256) void k_scs_build(
           for (int e = 0; e < 6; ++e) out[g][e] = (T)(acc[g][e] + (e < 3 ? inv : 0.0));
         }
       } else {
-        Comp6 acc[F];
+        // FP32 terms summed in FP64: a conversion and an add per term
+        // against the eight operations of a compensated FP32 sum, with the
+        // same registers and a smaller error (FP64 adds run at the FP32 rate)
+        double acc[F][6];
 #pragma unroll
-        for (int g = 0; g < F; ++g) acc[g].zero();
+        for (int g = 0; g < F; ++g)
+#pragma unroll
+          for (int e = 0; e < 6; ++e) acc[g][e] = 0.0;
         const float fb = (float)beta, fR = (float)R12;
-        float fs[F];
+        const float ibR = 1.0f / (fb * fR);
+        // per grid point 1 / sigma and 2/sqrt(pi) / sigma^3, per image the
+        // powers of 1 / r: one division per image instead of three per image
+        // and grid point (the hardware reciprocal estimate, biased, had
+        // raised the molecular C6 error tenfold)
+        float isg[F], zc[F];
 #pragma unroll
-        for (int g = 0; g < F; ++g) fs[g] = (float)sg12[g];
+        for (int g = 0; g < F; ++g) {
+          isg[g] = 1.0f / (float)sg12[g];
+          zc[g] = 2.25675833f * isg[g] * isg[g] * isg[g];
+        }
         for (int sidx = 0; sidx < nimg; ++sidx) {
           if (diag && sidx == center) continue;
           double dx, dy, dz, r;
@@ -224,37 +220,30 @@ __global__ __launch_bounds__(256) void k_scs_build(
           // frequency-independent parts: products, powers of r, damping
           const float x0 = (float)dx, x1 = (float)dy, x2 = (float)dz, rf = (float)r;
           const float r2 = rf * rf;
-          const float r5 = r2 * r2 * rf;
-          const float ef = __expf(-6.0f * (rf / (fb * fR) - 1.0f));
+          const float ir = 1.0f / rf, ir2 = ir * ir, ir5 = ir2 * ir2 * ir;
+          const float ef = __expf(-6.0f * (rf * ibR - 1.0f));
           const float damp = isinf(ef) ? 1.0f : ef / (1.0f + ef);
           const float dd[6] = {x0 * x0, x1 * x1, x2 * x2, x0 * x1, x0 * x2, x1 * x2};
 #pragma unroll
           for (int g = 0; g < F; ++g) {
             if (g < nfg) {
-              const float sg = fs[g];
-              const float x = rf / sg;
+              const float x = rf * isg[g];
               const float ex = __expf(-(x * x));
-              const float zl = (erff(x) - 1.12837917f * x * ex) / r5;
-              const float zr = (2.25675833f * ex) / (sg * sg * sg * r2);
+              const float zl = (erff(x) - 1.12837917f * x * ex) * ir5;
+              const float zr = ex * zc[g] * ir2;
 #pragma unroll
               for (int e = 0; e < 6; ++e) {
                 const float t = 3.0f * dd[e] - (e < 3 ? r2 : 0.0f);
-                acc[g].add(e, (-t * zl + dd[e] * zr) * damp);
+                acc[g][e] += (double)((-t * zl + dd[e] * zr) * damp);
               }
             }
           }
         }
 #pragma unroll
         for (int g = 0; g < F; ++g) {
-          if (diag) {
-            const double inv = 1.0 / alpha_w[(long)(f0 + min(g, nfg - 1)) * N + i];
+          const double inv = diag ? 1.0 / alpha_w[(long)(f0 + min(g, nfg - 1)) * N + i] : 0.0;
 #pragma unroll
-            for (int e = 0; e < 6; ++e)
-              out[g][e] = (T)((double)acc[g].s[e] + (double)acc[g].c[e] + (e < 3 ? inv : 0.0));
-          } else {
-#pragma unroll
-            for (int e = 0; e < 6; ++e) out[g][e] = (T)acc[g].get(e);
-          }
+          for (int e = 0; e < 6; ++e) out[g][e] = (T)(acc[g][e] + (e < 3 ? inv : 0.0));
         }
       }
     }
