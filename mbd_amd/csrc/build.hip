@@ -171,19 +171,22 @@ __global__ __launch_bounds__(256) void k_scs_build(
         sg12[g] = diag ? si : sqrt(si * si + sj * sj);
       }
       if (sizeof(T) == 8) {
-        double acc[F][6];
+        double acc[F][6], isg[F];
 #pragma unroll
-        for (int g = 0; g < F; ++g)
+        for (int g = 0; g < F; ++g) {
+          isg[g] = 1.0 / sg12[g];
 #pragma unroll
           for (int e = 0; e < 6; ++e) acc[g][e] = 0.0;
+        }
         for (int sidx = 0; sidx < nimg; ++sidx) {
           if (diag && sidx == center) continue;
           double dx, dy, dz, r;
           add3(d0x, d0y, d0z, img[3 * sidx], img[3 * sidx + 1], img[3 * sidx + 2], dx, dy, dz);
           if (!in_sphere(dx, dy, dz, rc, rc2x, r)) continue;
+          const double ir = 1.0 / r, damp = sr_damp64(r, R12, beta);
 #pragma unroll
           for (int g = 0; g < F; ++g)
-            if (g < nfg) sr_tensor_add64(dx, dy, dz, r, sg12[g], R12, beta, acc[g]);
+            if (g < nfg) sr_tensor_add64h(dx, dy, dz, r, ir, isg[g], damp, acc[g]);
         }
 #pragma unroll
         for (int g = 0; g < F; ++g) {

@@ -48,20 +48,46 @@ __device__ inline void sr_tensor_add64(double dx, double dy, double dz,
   }
 }
 
+// The SR tensor with the Fermi damping, 1 / r and 1 / sgm given: the callers
+// hoist them out of their loops over images and grid points (one division per
+// image and per grid point instead of five per term).
+__device__ inline double sr_damp64(double r, double rvdw, double beta) {
+  const double fermi = r / (beta * rvdw);
+  return 1.0 - (1.0 / (1.0 + exp(-6.0 * (fermi - 1.0))));
+}
+__device__ inline void sr_tensor_add64h(double dx, double dy, double dz, double r, double ir,
+                                        double isg, double damp, double *acc) {
+  const double x = r * isg;
+  const double ex = exp(-(x * x));
+  const double r2 = r * r, ir2 = ir * ir;
+  const double zl = (erf(x) - sqrt(4.0 / kPi) * x * ex) * (ir2 * ir2 * ir);
+  const double zr = (sqrt(16.0 / kPi) * ex) * (isg * isg * isg * ir2);
+  const double d[3] = {dx, dy, dz};
+  const int rr[6] = {0, 1, 2, 0, 0, 1}, cc[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    const double dd = d[rr[e]] * d[cc[e]];
+    const double t = 3.0 * dd - (e < 3 ? r2 : 0.0);
+    acc[e] += (-t * zl + dd * zr) * damp;
+  }
+}
+
 // LR tensor (UTILS.f90:1140-1150) times pref, accumulated.
 __device__ inline void lr_tensor_add64(double dx, double dy, double dz,
                                        double r, double rvdw, double beta,
                                        double pref, double *acc) {
   const double fermi = r / (beta * rvdw);
-  const double f = 1.0 / (1.0 + exp(-6.0 * (fermi - 1.0)));
   const double r2 = r * r;
   const double r5 = r2 * r2 * r;
+  // the Fermi damping, 1 / r^5 and the prefactor in one division (instead of
+  // one per element and one for the damping: divisions are most of the work)
+  const double c = pref / ((1.0 + exp(-6.0 * (fermi - 1.0))) * r5);
   const double d[3] = {dx, dy, dz};
   const int rr[6] = {0, 1, 2, 0, 0, 1}, cc[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
   for (int e = 0; e < 6; ++e) {
     const double t = -3.0 * d[rr[e]] * d[cc[e]] + (e < 3 ? r2 : 0.0);
-    acc[e] += ((t / r5) * f) * pref;
+    acc[e] += t * c;
   }
 }
 
