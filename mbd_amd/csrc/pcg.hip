@@ -230,9 +230,14 @@ Clusters build_clusters(int N, const double *xyz, const double *lat, bool period
 // ---- device ---------------------------------------------------------------
 
 // Inverse of each cluster's diagonal block of matrix blockIdx.y (gathered
-// from the lower triangle, Cholesky and two triangular solves in LDS, FP64).
-// A block that is not positive definite marks its matrix as indefinite (the
-// iteration is skipped for it) and gets the inverse of its diagonal.
+// from the lower triangle, FP64) by Gauss-Jordan elimination without
+// pivoting, in registers: thread (bi, bj) of a 16 x 16 grid holds the 6 x 6
+// sub-block at rows 6 bi, columns 6 bj of the (at most 96 x 96) block, and
+// step k shares row and column k through LDS (double-buffered: one barrier per
+// step).  The pivots are those of the block's LDL^T factorisation: a block
+// that is not positive definite (a pivot not above zero) marks its matrix as
+// indefinite (the iteration is skipped for it) and gets the inverse of its
+// diagonal.
 template <typename T>
 __global__ __launch_bounds__(256) void k_cl_setup(const T *__restrict__ A, int lda, long sA,
                                                   const int *__restrict__ off,
@@ -240,40 +245,73 @@ __global__ __launch_bounds__(256) void k_cl_setup(const T *__restrict__ A, int l
                                                   const long *__restrict__ moff, long mtot,
                                                   double *__restrict__ Minv,
                                                   double *__restrict__ scal) {
-  __shared__ double L[MB][MB + 1];
-  __shared__ double Y[MB][MB + 1];
-  __shared__ int rows[MB];
-  __shared__ int bad;
+  constexpr int SB = 6;  // sub-block of a thread
+  static_assert(16 * SB >= MB, "a 16 x 16 grid of 6 x 6 sub-blocks covers the largest block");
+  __shared__ double rowk[2][16 * SB], colk[2][16 * SB];
+  __shared__ int rows[16 * SB];
   const int c = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
   const int a0 = off[c], s = off[c + 1] - a0, m = 3 * s;
   const T *M = A + (long)b * sA;
-  if (t < m) rows[t] = 3 * atoms[a0 + t / 3] + t % 3;
-  if (t == 0) bad = 0;
+  const int bi = t / 16, bj = t % 16, i0 = SB * bi, j0 = SB * bj;
+  if (t < 16 * SB) rows[t] = (t < m) ? 3 * atoms[a0 + t / 3] + t % 3 : 0;
   __syncthreads();
-  for (int e = t; e < m * m; e += 256) {
-    const int r = e % m, q = e / m;
-    const int gr = rows[r], gc = rows[q];
-    L[q][r] = (double)(gr >= gc ? M[gr + (long)gc * lda] : M[gc + (long)gr * lda]);
-  }
-  __syncthreads();
-  // right-looking Cholesky, L[col][row] holds the lower factor
-  for (int j = 0; j < m; ++j) {
-    const double d = L[j][j];
-    if (!(d > 0.0)) {
-      if (t == 0) bad = 1;
+  double x[SB][SB];
+#pragma unroll
+  for (int ii = 0; ii < SB; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < SB; ++jj) {
+      const int r = i0 + ii, q = j0 + jj;
+      double v = 0.0;
+      if (r < m && q < m) {
+        const int gr = rows[r], gc = rows[q];
+        v = (double)(gr >= gc ? M[gr + (long)gc * lda] : M[gc + (long)gr * lda]);
+      }
+      x[ii][jj] = v;
+    }
+  bool bad = false;
+  for (int k = 0; k < m; ++k) {
+    const int bf = k & 1, kb = k / SB, kk = k % SB;
+    // the owners of row k and of column k publish them (values before the step)
+    if (bi == kb) {
+#pragma unroll
+      for (int jj = 0; jj < SB; ++jj) {
+        double v = x[0][jj];
+#pragma unroll
+        for (int q = 1; q < SB; ++q) v = (kk == q) ? x[q][jj] : v;
+        rowk[bf][j0 + jj] = v;
+      }
+    }
+    if (bj == kb) {
+#pragma unroll
+      for (int ii = 0; ii < SB; ++ii) {
+        double v = x[ii][0];
+#pragma unroll
+        for (int q = 1; q < SB; ++q) v = (kk == q) ? x[ii][q] : v;
+        colk[bf][i0 + ii] = v;
+      }
+    }
+    __syncthreads();
+    const double p = rowk[bf][k];
+    if (!(p > 0.0)) {  // the same value in every thread
+      bad = true;
       break;
     }
-    const double ljj = sqrt(d);
-    __syncthreads();
-    for (int r = j + t; r < m; r += 256) L[j][r] = (r == j) ? ljj : L[j][r] / ljj;
-    __syncthreads();
-    for (int e = t; e < (m - j - 1) * (m - j - 1); e += 256) {
-      const int r = j + 1 + e % (m - j - 1), q = j + 1 + e / (m - j - 1);
-      if (r >= q) L[q][r] -= L[j][r] * L[j][q];
+    const double ip = 1.0 / p;
+    double ck[SB], rk[SB];
+#pragma unroll
+    for (int q = 0; q < SB; ++q) {
+      ck[q] = colk[bf][i0 + q];
+      rk[q] = rowk[bf][j0 + q];
     }
-    __syncthreads();
+#pragma unroll
+    for (int ii = 0; ii < SB; ++ii)
+#pragma unroll
+      for (int jj = 0; jj < SB; ++jj) {
+        const bool ri = (i0 + ii == k), cj = (j0 + jj == k);
+        const double v = x[ii][jj];
+        x[ii][jj] = (ri && cj) ? ip : ri ? v * ip : cj ? -v * ip : fma(-ck[ii] * ip, rk[jj], v);
+      }
   }
-  __syncthreads();
   double *out = Minv + (long)b * mtot + moff[c];
   if (bad) {
     if (t == 0) scal[(long)b * kPcgScal + S_INDEF] = 1.0;
@@ -284,22 +322,14 @@ __global__ __launch_bounds__(256) void k_cl_setup(const T *__restrict__ A, int l
     }
     return;
   }
-  // column q of the inverse: L y = e_q, then L^T z = y (thread q)
-  if (t < m) {
-    const int q = t;
-    for (int r = 0; r < m; ++r) {
-      double v = (r == q) ? 1.0 : 0.0;
-      for (int k = q; k < r; ++k) v -= L[k][r] * Y[q][k];
-      Y[q][r] = (r < q) ? 0.0 : v / L[r][r];
+  // out[q m + r] = inverse[r][q]
+#pragma unroll
+  for (int ii = 0; ii < SB; ++ii)
+#pragma unroll
+    for (int jj = 0; jj < SB; ++jj) {
+      const int r = i0 + ii, q = j0 + jj;
+      if (r < m && q < m) out[(long)q * m + r] = x[ii][jj];
     }
-    for (int r = m - 1; r >= 0; --r) {
-      double v = Y[q][r];
-      for (int k = r + 1; k < m; ++k) v -= L[r][k] * Y[q][k];
-      Y[q][r] = v / L[r][r];
-    }
-  }
-  __syncthreads();
-  for (int e = t; e < m * m; e += 256) out[e] = Y[e / m][e % m];
 }
 
 // X0 = M^-1 B per cluster, the solution with only intra-cluster screening.
