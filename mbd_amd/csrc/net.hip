@@ -43,8 +43,13 @@ __global__ __launch_bounds__(256) void k_net_nbl(int N, const double *__restrict
     int k = 0;
     int base = 0;
     if (pass == 1) base = pre[t];
-    for (long e = e0; e < e1; ++e) {
-      const int j = (int)(e / nimg), s = (int)(e % nimg);
+    // (j, image) of candidate e = j nimg + s, stepped instead of divided
+    int j = (int)(e0 / nimg), s = (int)(e0 % nimg);
+    for (long e = e0; e < e1; ++e, ++s) {
+      if (s == nimg) {
+        s = 0;
+        ++j;
+      }
       if (j == i && s == center) continue;
       const double dx = xi - xyz[3 * j] + img[3 * s];
       const double dy = yi - xyz[3 * j + 1] + img[3 * s + 1];
