@@ -962,6 +962,23 @@ int mbd_set_system(mbd_handle *h, int n_atoms, const double *xyz, const double *
         if (L <= kNetCut + diam || (int)q == h->nimg_scs / 2)
           for (int c = 0; c < 3; ++c) tn.push_back(ti[3 * q + c]);
       }
+      // sorted by |L| (stable), so that the neighbour scan can bound, per atom
+      // pair, the images that can come within the cutoff
+      {
+        const int nt = (int)(tn.size() / 3);
+        std::vector<int> ord(nt);
+        for (int q = 0; q < nt; ++q) ord[q] = q;
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](int a, int b) { return norm3(&tn[3 * a]) < norm3(&tn[3 * b]); });
+        std::vector<double> ts(tn.size());
+        int cnew = -1;
+        for (int q = 0; q < nt; ++q) {
+          for (int c = 0; c < 3; ++c) ts[3 * q + c] = tn[3 * ord[q] + c];
+          if (ord[q] == h->center_net) cnew = q;
+        }
+        tn.swap(ts);
+        h->center_net = cnew;
+      }
       h->nimg_net = (int)(tn.size() / 3);
       up(h->img_net, tn.data(), tn.size());
     }

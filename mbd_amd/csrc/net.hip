@@ -25,17 +25,42 @@ namespace mbd {
 
 // Neighbour list of atom i (one workgroup per atom): every (j, image) with
 // 0 < r < kNetCut, in (j, image) order via a two-pass ordered compaction;
-// at most kNetCap entries (float4: dx, dy, dz, j).
+// at most kNetCap entries (float4: dx, dy, dz, j).  The images come sorted by
+// |L|: for a pair at distance d0 in the cell only images with
+// d0 - kNetCut < |L| < d0 + kNetCut can come within the cutoff, a range found
+// by bisection over the norms in LDS (up to kImgLds images; more are scanned
+// in full).
 __global__ __launch_bounds__(256) void k_net_nbl(int N, const double *__restrict__ xyz, int nimg,
                                                  int center, const double *__restrict__ img,
                                                  float4 *__restrict__ nbl, int *__restrict__ cnt) {
+  constexpr int kImgLds = 2048;
   __shared__ int pre[256];
+  __shared__ double lnorm[kImgLds];
   const int i = blockIdx.x, t = threadIdx.x;
   const double rc2 = kNetCut * kNetCut;
   const double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
-  const long total = (long)N * nimg;
+  const bool bounded = nimg <= kImgLds;
+  if (bounded)
+    for (int q = t; q < nimg; q += blockDim.x)
+      lnorm[q] = sqrt(img[3 * q] * img[3 * q] + img[3 * q + 1] * img[3 * q + 1] +
+                      img[3 * q + 2] * img[3 * q + 2]);
+  __syncthreads();
+  // first image whose norm is above v (norms ascending)
+  auto first_above = [&](double v) {
+    int lo = 0, hi = nimg;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (lnorm[mid] <= v)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    return lo;
+  };
   // each thread owns a contiguous range of candidates so that the compaction
-  // preserves the (j, image) order
+  // preserves the (j, image) order: atoms j when the images are bounded,
+  // (j, image) pairs otherwise
+  const long total = bounded ? (long)N : (long)N * nimg;
   const long per = (total + blockDim.x - 1) / blockDim.x;
   const long e0 = t * per, e1 = (e0 + per < total) ? e0 + per : total;
   int hits = 0;
@@ -43,22 +68,36 @@ __global__ __launch_bounds__(256) void k_net_nbl(int N, const double *__restrict
     int k = 0;
     int base = 0;
     if (pass == 1) base = pre[t];
-    // (j, image) of candidate e = j nimg + s, stepped instead of divided
-    int j = (int)(e0 / nimg), s = (int)(e0 % nimg);
-    for (long e = e0; e < e1; ++e, ++s) {
-      if (s == nimg) {
-        s = 0;
-        ++j;
-      }
-      if (j == i && s == center) continue;
-      const double dx = xi - xyz[3 * j] + img[3 * s];
-      const double dy = yi - xyz[3 * j + 1] + img[3 * s + 1];
-      const double dz = zi - xyz[3 * j + 2] + img[3 * s + 2];
+    auto test = [&](int j, int s, double d0x, double d0y, double d0z) {
+      if (j == i && s == center) return;
+      const double dx = d0x + img[3 * s];
+      const double dy = d0y + img[3 * s + 1];
+      const double dz = d0z + img[3 * s + 2];
       const double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 >= rc2 || r2 == 0.0) continue;
+      if (r2 >= rc2 || r2 == 0.0) return;
       if (pass == 1 && base + k < kNetCap)
         nbl[(long)i * kNetCap + base + k] = make_float4((float)dx, (float)dy, (float)dz, (float)j);
       ++k;
+    };
+    if (bounded) {
+      for (long e = e0; e < e1; ++e) {
+        const int j = (int)e;
+        const double d0x = xi - xyz[3 * j], d0y = yi - xyz[3 * j + 1], d0z = zi - xyz[3 * j + 2];
+        const double dn = sqrt(d0x * d0x + d0y * d0y + d0z * d0z);
+        // a slack of 1e-6 bohr against the rounding of the norms
+        const int s0 = first_above(dn - kNetCut - 1e-6), s1 = first_above(dn + kNetCut + 1e-6);
+        for (int s = s0; s < s1; ++s) test(j, s, d0x, d0y, d0z);
+      }
+    } else {
+      // (j, image) of candidate e = j nimg + s, stepped instead of divided
+      int j = (int)(e0 / nimg), s = (int)(e0 % nimg);
+      for (long e = e0; e < e1; ++e, ++s) {
+        if (s == nimg) {
+          s = 0;
+          ++j;
+        }
+        test(j, s, xi - xyz[3 * j], yi - xyz[3 * j + 1], zi - xyz[3 * j + 2]);
+      }
     }
     if (pass == 0) {
       hits = k;
