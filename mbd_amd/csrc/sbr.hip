@@ -898,74 +898,61 @@ __global__ __launch_bounds__(256) void k_sbr_symm(int m, const double *__restric
   const int r0 = blockIdx.x * SY_R;
   const int k0 = blockIdx.y * kchunk, k1 = min(m, k0 + kchunk);
   v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-  // two register sets: a chunk is loaded two chunks ahead of its use (one
-  // chunk ahead left about 48 KB in flight per CU, which bounded the product
-  // by memory latency)
-  double ra[2][4], rx[2][2];
-  int kcs[2] = {0, 0};
-  // raw values at clamped addresses; the mask is applied at the LDS store (a
-  // load under a lane condition, or a select right after it, is waited for
-  // where it stands)
-  auto gload = [&](int kc, int k) {
+  double ra[4], rx[2];
+  bool up_cur = false;
+  auto gload = [&](int kc) {
     const bool upper = kc >= r0 + SY_R;  // chunk right of the row block: A22[r][c] = A[c][r]
-    kcs[k] = kc;
+    up_cur = upper;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int e = t + 256 * i;
       const int rr = upper ? (e >> 4) : (e & 63), kk = upper ? (e & 15) : (e >> 6);
-      const int gr = min(r0 + rr, m - 1), gc = min(kc + kk, k1 - 1);
-      ra[k][i] = A[(gr >= gc) ? gr + (long)gc * lda : gc + (long)gr * lda];
+      const int gr = r0 + rr, gc = kc + kk;
+      double v = 0.0;
+      if (gr < m && gc < k1) v = (gr >= gc) ? A[gr + (long)gc * lda] : A[gc + (long)gr * lda];
+      ra[i] = v;
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int e = t + 256 * i, kk = e & 15, cc = e >> 4;
-      rx[k][i] = X[min(kc + kk, k1 - 1) + (long)cc * ldx];
+      const int gc = kc + kk;
+      rx[i] = (gc < k1) ? X[gc + (long)cc * ldx] : 0.0;
     }
   };
-  auto sstore = [&](int buf, int k) {
-    const int kc = kcs[k];
-    const bool upper = kc >= r0 + SY_R;
+  auto sstore = [&](int buf) {
+    const bool upper = up_cur;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int e = t + 256 * i;
       const int rr = upper ? (e >> 4) : (e & 63), kk = upper ? (e & 15) : (e >> 6);
-      As[buf][kk][rr] = (r0 + rr < m && kc + kk < k1) ? ra[k][i] : 0.0;
+      As[buf][kk][rr] = ra[i];
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const int e = t + 256 * i, kk = e & 15, cc = e >> 4;
-      Xs[buf][kk][cc] = (kc + kk < k1) ? rx[k][i] : 0.0;
+      Xs[buf][kk][cc] = rx[i];
     }
   };
   const int nk = (k1 > k0) ? (k1 - k0 + SY_K - 1) / SY_K : 0;
-  const int klast = k0 + SY_K * max(nk - 1, 0);
   if (nk > 0) {
-    gload(k0, 0);
-    gload(min(k0 + SY_K, klast), 1);
-    sstore(0, 0);
+    gload(k0);
+    sstore(0);
   }
   __syncthreads();
-  // chunk it from LDS buffer it & 1; register set it & 1 receives chunk it + 2
-  // (the last chunk again past the end) and set (it + 1) & 1 holds chunk it + 1
-  auto step = [&](int it, int k) {
-    gload(min(k0 + (it + 2) * SY_K, klast), k);
+  for (int it = 0; it < nk; ++it) {
+    const int cur = it & 1;
+    if (it + 1 < nk) gload(k0 + (it + 1) * SY_K);
 #pragma unroll
     for (int kk = 0; kk < SY_K; kk += 4) {
-      const double a = As[k][kk + lk][16 * wv + lr];
-      const double b0 = Xs[k][kk + lk][lr], b1 = Xs[k][kk + lk][16 + lr];
+      const double a = As[cur][kk + lk][16 * wv + lr];
+      const double b0 = Xs[cur][kk + lk][lr], b1 = Xs[cur][kk + lk][16 + lr];
       // transposed blocks: lane (lk, lr) accumulates Y[row 16 wv + lr][col lk + 4 v]
       acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a, acc0, 0, 0, 0);
       acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a, acc1, 0, 0, 0);
     }
-    if (it + 1 < nk) sstore(k ^ 1, k ^ 1);
+    if (it + 1 < nk) sstore(cur ^ 1);
     __syncthreads();
-  };
-  int it = 0;
-  for (; it + 1 < nk; it += 2) {
-    step(it, 0);
-    step(it + 1, 1);
   }
-  if (it < nk) step(it, 0);
   double *Y = Yp + (long)blockIdx.y * B * ldy;
   const int row = r0 + 16 * wv + lr;
   if (row < m) {
