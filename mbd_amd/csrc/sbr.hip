@@ -984,6 +984,19 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
   const int r = t & 63, cg = t >> 6;
   const int row = 64 * I + r;
   const bool pend = Vq != nullptr;
+  // the pending V and Z rows, loaded with the rest (unconditionally, from X
+  // when there is no pending update) instead of after the first product
+  double qv[2][8];
+  {
+    const double *Q0 = pend ? Vq : X, *Q1 = pend ? Zq : X;
+    const long lq = pend ? ldq : ldx;
+    const int rr = min(row, m - 1);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      qv[0][j] = Q0[rr + (long)(8 * cg + j) * lq];
+      qv[1][j] = Q1[rr + (long)(8 * cg + j) * lq];
+    }
+  }
   {
     // every load first, then the stores: a store into Yp between the loads
     // would order each column's loads behind the previous column's store.
@@ -1027,14 +1040,11 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
   tile(Xs, Ys, Mp + (long)I * NQ);
   if (pend) {
     // reuse Ys for the pending V block, then the pending Z block
+#pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
-      const double *Q = pass ? Zq : Vq;
       __syncthreads();
-      double qv[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) qv[j] = Q[min(row, m - 1) + (long)(8 * cg + j) * ldq];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) Ys[8 * cg + j][r] = (row < m) ? qv[j] : 0.0;
+      for (int j = 0; j < 8; ++j) Ys[8 * cg + j][r] = (row < m) ? qv[pass][j] : 0.0;
       __syncthreads();
       tile(Ys, Xs, Mp + (long)I * NQ + (1 + pass) * B * B);
     }
