@@ -1096,6 +1096,7 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
         for (int g = 0; g < G; ++g) acc[U * r + u] += (g0 + g < ng) ? v[u][g] : 0.0;
     }
   }
+  double *mx = &Ys[0][0] + B * B;  // X^T Y beside GV (Ys holds 32 x 65)
 #pragma unroll
   for (int i = 0; i < NQ / 256; ++i) {
     const int e = t + 256 * i;
@@ -1104,18 +1105,25 @@ __global__ __launch_bounds__(256) void k_sbr_zred(int m, int nsplit, double *__r
       gz[e - 2 * B * B] = acc[i];
     else if (e >= B * B)
       gv[e - B * B] = acc[i];
+    else if (pend)
+      mx[e] = acc[i];
     if (!pend || e >= B * B) M[e] = acc[i];
   }
   if (pend) {
     __syncthreads();
-    // M = X^T Y - GV^T GZ - GZ^T GV (row-major 32 x 32 blocks)
+    // M = X^T Y - (GV^T GZ + GZ^T GV) (row-major 32 x 32 blocks), the two
+    // products on MFMA, wave v owning the 16 x 16 block (v >> 1, v & 1)
+    v4d c = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int e = t + 256 * i, a = e >> 5, b = e & 31;
-      double s = 0.0;
-#pragma unroll 8
-      for (int l = 0; l < B; ++l) s += gv[l * B + a] * gz[l * B + b] + gz[l * B + a] * gv[l * B + b];
-      M[e] = acc[i] - s;
+    for (int k4 = 0; k4 < B / 4; ++k4) {
+      const int l = 4 * k4 + lk;
+      c = __builtin_amdgcn_mfma_f64_16x16x4f64(gv[l * B + 16 * ti + lr], gz[l * B + 16 * tj + lr], c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f64_16x16x4f64(gz[l * B + 16 * ti + lr], gv[l * B + 16 * tj + lr], c, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int e = (16 * ti + lk + 4 * v) * B + 16 * tj + lr;
+      M[e] = mx[e] - c[v];
     }
   }
 }
