@@ -562,16 +562,17 @@ __global__ __launch_bounds__(256) void k_gersh(int n, const double *d,
   }
 }
 
-// Multisection: 8 lanes share one eigenvalue index k and evaluate Sturm
-// counts at 8 interior points of the current interval; the group keeps the
-// sub-interval that holds eigenvalue k (9-fold reduction per sweep) until
-// the interval is within 2 ulp (+ pivmin) of the eigenvalue.  Every
-// eigenvalue starts from one cell of an equispaced grid of kBisectGrid cells
-// over the Gershgorin interval, whose Sturm counts one pass computes for all
-// eigenvalues at once (about five sweeps fewer per eigenvalue).  Eight lanes
-// measured slightly faster than sixteen (fewer counts per eigenvalue; the
-// division-bound recurrence keeps the SIMDs busy with either).
-constexpr int kSect = 8;
+// Multisection: kSect lanes share one eigenvalue index k and evaluate Sturm
+// counts at kSect interior points of the current interval; the group keeps
+// the sub-interval that holds eigenvalue k ((kSect + 1)-fold reduction per
+// sweep) until the interval is within 2 ulp (+ pivmin) of the eigenvalue.
+// Every eigenvalue starts from one cell of an equispaced grid of kBisectGrid
+// cells over the Gershgorin interval, whose Sturm counts one pass computes
+// for all eigenvalues at once.  Measured at n = 12288 (whole eigen stage):
+// 16 lanes 216.7 ms, 8 lanes 214.8, 4 lanes 210.2, 2 lanes 215.4: fewer
+// counts per eigenvalue until too few waves are left to hide the
+// recurrence's latency.
+constexpr int kSect = 4;
 constexpr int kBisectMaxSweeps = 40;
 
 __device__ inline int sturm_count(int n, const double *__restrict__ d,
