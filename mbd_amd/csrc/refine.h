@@ -19,7 +19,7 @@ constexpr double kPcgGate = 1e-9;     // true residual above which PCG falls bac
 // kSvW columns walked down in tiles of kSvH rows, kSvSeg tiles per segment.
 constexpr int kSvW = 256;
 constexpr int kSvH = 32;
-constexpr int kSvSeg = 64;
+constexpr int kSvSeg = 24;
 struct Symv3Sizes {
   int ntile, nstrip, nseg, maxseg;
   size_t upart, vpart;  // doubles of the row / column partials per matrix
