@@ -883,7 +883,7 @@ __global__ __launch_bounds__(NT) void k_sbr_cq3(int m, const double *__restrict_
 }
 
 namespace {
-constexpr int SY_R = 64, SY_K = 16;
+constexpr int SY_R = 64, SY_K = 32;
 }
 
 // Partial Y = A22[rows, cols] X[cols] for row block blockIdx.x (64 rows) and
@@ -892,66 +892,66 @@ constexpr int SY_R = 64, SY_K = 16;
 __global__ __launch_bounds__(256) void k_sbr_symm(int m, const double *__restrict__ A, long lda,
                                                   const double *__restrict__ X, long ldx,
                                                   double *__restrict__ Yp, long ldy, int kchunk) {
-  __shared__ double As[2][SY_K][SY_R + 1];
-  __shared__ double Xs[2][SY_K][B + 1];
+  // one LDS buffer of SY_K = 32 columns: twice the bytes in flight per
+  // workgroup of a double-buffered 16-column chunk in the same LDS, at two
+  // barriers per chunk
+  __shared__ double As[SY_K][SY_R + 1];
+  __shared__ double Xs[SY_K][B + 1];
+  constexpr int NA = SY_K * SY_R / 256, NX = SY_K * B / 256;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 15, lk = lane >> 4;
   const int r0 = blockIdx.x * SY_R;
   const int k0 = blockIdx.y * kchunk, k1 = min(m, k0 + kchunk);
   v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-  double ra[4], rx[2];
+  double ra[NA], rx[NX];
   bool up_cur = false;
   auto gload = [&](int kc) {
     const bool upper = kc >= r0 + SY_R;  // chunk right of the row block: A22[r][c] = A[c][r]
     up_cur = upper;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < NA; ++i) {
       const int e = t + 256 * i;
-      const int rr = upper ? (e >> 4) : (e & 63), kk = upper ? (e & 15) : (e >> 6);
+      const int rr = upper ? (e / SY_K) : (e & 63), kk = upper ? (e % SY_K) : (e >> 6);
       const int gr = r0 + rr, gc = kc + kk;
       double v = 0.0;
       if (gr < m && gc < k1) v = (gr >= gc) ? A[gr + (long)gc * lda] : A[gc + (long)gr * lda];
       ra[i] = v;
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int e = t + 256 * i, kk = e & 15, cc = e >> 4;
+    for (int i = 0; i < NX; ++i) {
+      const int e = t + 256 * i, kk = e % SY_K, cc = e / SY_K;
       const int gc = kc + kk;
       rx[i] = (gc < k1) ? X[gc + (long)cc * ldx] : 0.0;
     }
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&]() {
     const bool upper = up_cur;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < NA; ++i) {
       const int e = t + 256 * i;
-      const int rr = upper ? (e >> 4) : (e & 63), kk = upper ? (e & 15) : (e >> 6);
-      As[buf][kk][rr] = ra[i];
+      const int rr = upper ? (e / SY_K) : (e & 63), kk = upper ? (e % SY_K) : (e >> 6);
+      As[kk][rr] = ra[i];
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int e = t + 256 * i, kk = e & 15, cc = e >> 4;
-      Xs[buf][kk][cc] = rx[i];
+    for (int i = 0; i < NX; ++i) {
+      const int e = t + 256 * i, kk = e % SY_K, cc = e / SY_K;
+      Xs[kk][cc] = rx[i];
     }
   };
   const int nk = (k1 > k0) ? (k1 - k0 + SY_K - 1) / SY_K : 0;
-  if (nk > 0) {
-    gload(k0);
-    sstore(0);
-  }
-  __syncthreads();
+  if (nk > 0) gload(k0);
   for (int it = 0; it < nk; ++it) {
-    const int cur = it & 1;
+    __syncthreads();  // the previous chunk's LDS reads are done
+    sstore();
+    __syncthreads();
     if (it + 1 < nk) gload(k0 + (it + 1) * SY_K);
 #pragma unroll
     for (int kk = 0; kk < SY_K; kk += 4) {
-      const double a = As[cur][kk + lk][16 * wv + lr];
-      const double b0 = Xs[cur][kk + lk][lr], b1 = Xs[cur][kk + lk][16 + lr];
+      const double a = As[kk + lk][16 * wv + lr];
+      const double b0 = Xs[kk + lk][lr], b1 = Xs[kk + lk][16 + lr];
       // transposed blocks: lane (lk, lr) accumulates Y[row 16 wv + lr][col lk + 4 v]
       acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a, acc0, 0, 0, 0);
       acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a, acc1, 0, 0, 0);
     }
-    if (it + 1 < nk) sstore(cur ^ 1);
-    __syncthreads();
   }
   double *Y = Yp + (long)blockIdx.y * B * ldy;
   const int row = r0 + 16 * wv + lr;
