@@ -58,6 +58,43 @@ __device__ inline bool in_sphere(double dx, double dy, double dz, double rc, dou
   return r <= rc;
 }
 
+// Image loops of the builders: a wave-uniform pass over a chunk of 32 images
+// (scalar loads) marks in a per-lane mask the images that may lie in the
+// sphere (rounded r^2 <= rc2x), other than image `skip`; each lane then runs
+// the tensor only over its own marked images (in_sphere decides exactly).  A
+// wave so runs the tensor once per marked image of its fullest pair in each
+// chunk rather than once per image marked by any of its 64 pairs (at a 100 A
+// cutoff in a 38 A cell about 76 of 343 images per pair, scattered
+// differently over the lanes).  Each pair still sums its images in table order.
+constexpr int kImgChunk = 32;
+__device__ inline unsigned sphere_mask(double d0x, double d0y, double d0z,
+                                       const double *__restrict__ img, int c0, int cn, int skip,
+                                       double rc2x) {
+  unsigned m = 0;
+  for (int b = 0; b < cn; ++b) {
+    double dx, dy, dz, s2;
+    add3(d0x, d0y, d0z, img[3 * (c0 + b)], img[3 * (c0 + b) + 1], img[3 * (c0 + b) + 2], dx, dy, dz);
+    {
+#pragma clang fp contract(off)
+      s2 = dx * dx + dy * dy;
+      s2 = s2 + dz * dz;
+    }
+    if (s2 <= rc2x && c0 + b != skip) m |= 1u << b;
+  }
+  return m;
+}
+
+// Next marked image of a mask (cleared from it): its displacement and, if it
+// lies in the sphere, r; false if it does not.
+__device__ inline bool take_marked(double d0x, double d0y, double d0z,
+                                   const double *__restrict__ img, int c0, unsigned &m, double rc,
+                                   double rc2x, double &dx, double &dy, double &dz, double &r) {
+  const int s = c0 + __builtin_ctz(m);
+  m &= m - 1;
+  add3(d0x, d0y, d0z, img[3 * s], img[3 * s + 1], img[3 * s + 2], dx, dy, dz);
+  return in_sphere(dx, dy, dz, rc, rc2x, r);
+}
+
 // Six unique entries of a symmetric 3x3: xx, yy, zz, xy, xz, yz.
 template <typename T>
 struct Sym6 {
@@ -178,15 +215,17 @@ __global__ __launch_bounds__(256) void k_scs_build(
 #pragma unroll
           for (int e = 0; e < 6; ++e) acc[g][e] = 0.0;
         }
-        for (int sidx = 0; sidx < nimg; ++sidx) {
-          if (diag && sidx == center) continue;
-          double dx, dy, dz, r;
-          add3(d0x, d0y, d0z, img[3 * sidx], img[3 * sidx + 1], img[3 * sidx + 2], dx, dy, dz);
-          if (!in_sphere(dx, dy, dz, rc, rc2x, r)) continue;
-          const double ir = 1.0 / r, damp = sr_damp64(r, R12, beta);
+        for (int c0 = 0; c0 < nimg; c0 += kImgChunk) {
+          unsigned m = sphere_mask(d0x, d0y, d0z, img, c0, min(kImgChunk, nimg - c0),
+                                   diag ? center : -1, rc2x);
+          while (m) {
+            double dx, dy, dz, r;
+            if (!take_marked(d0x, d0y, d0z, img, c0, m, rc, rc2x, dx, dy, dz, r)) continue;
+            const double ir = 1.0 / r, damp = sr_damp64(r, R12, beta);
 #pragma unroll
-          for (int g = 0; g < F; ++g)
-            if (g < nfg) sr_tensor_add64h(dx, dy, dz, r, ir, isg[g], damp, acc[g]);
+            for (int g = 0; g < F; ++g)
+              if (g < nfg) sr_tensor_add64h(dx, dy, dz, r, ir, isg[g], damp, acc[g]);
+          }
         }
 #pragma unroll
         for (int g = 0; g < F; ++g) {
@@ -215,29 +254,31 @@ __global__ __launch_bounds__(256) void k_scs_build(
           isg[g] = 1.0f / (float)sg12[g];
           zc[g] = 2.25675833f * isg[g] * isg[g] * isg[g];
         }
-        for (int sidx = 0; sidx < nimg; ++sidx) {
-          if (diag && sidx == center) continue;
-          double dx, dy, dz, r;
-          add3(d0x, d0y, d0z, img[3 * sidx], img[3 * sidx + 1], img[3 * sidx + 2], dx, dy, dz);
-          if (!in_sphere(dx, dy, dz, rc, rc2x, r)) continue;
-          // frequency-independent parts: products, powers of r, damping
-          const float x0 = (float)dx, x1 = (float)dy, x2 = (float)dz, rf = (float)r;
-          const float r2 = rf * rf;
-          const float ir = 1.0f / rf, ir2 = ir * ir, ir5 = ir2 * ir2 * ir;
-          const float ef = __expf(-6.0f * (rf * ibR - 1.0f));
-          const float damp = isinf(ef) ? 1.0f : ef / (1.0f + ef);
-          const float dd[6] = {x0 * x0, x1 * x1, x2 * x2, x0 * x1, x0 * x2, x1 * x2};
+        for (int c0 = 0; c0 < nimg; c0 += kImgChunk) {
+          unsigned m = sphere_mask(d0x, d0y, d0z, img, c0, min(kImgChunk, nimg - c0),
+                                   diag ? center : -1, rc2x);
+          while (m) {
+            double dx, dy, dz, r;
+            if (!take_marked(d0x, d0y, d0z, img, c0, m, rc, rc2x, dx, dy, dz, r)) continue;
+            // frequency-independent parts: products, powers of r, damping
+            const float x0 = (float)dx, x1 = (float)dy, x2 = (float)dz, rf = (float)r;
+            const float r2 = rf * rf;
+            const float ir = 1.0f / rf, ir2 = ir * ir, ir5 = ir2 * ir2 * ir;
+            const float ef = __expf(-6.0f * (rf * ibR - 1.0f));
+            const float damp = isinf(ef) ? 1.0f : ef / (1.0f + ef);
+            const float dd[6] = {x0 * x0, x1 * x1, x2 * x2, x0 * x1, x0 * x2, x1 * x2};
 #pragma unroll
-          for (int g = 0; g < F; ++g) {
-            if (g < nfg) {
-              const float x = rf * isg[g];
-              const float ex = __expf(-(x * x));
-              const float zl = (erff(x) - 1.12837917f * x * ex) * ir5;
-              const float zr = ex * zc[g] * ir2;
+            for (int g = 0; g < F; ++g) {
+              if (g < nfg) {
+                const float x = rf * isg[g];
+                const float ex = __expf(-(x * x));
+                const float zl = (erff(x) - 1.12837917f * x * ex) * ir5;
+                const float zr = ex * zc[g] * ir2;
 #pragma unroll
-              for (int e = 0; e < 6; ++e) {
-                const float t = 3.0f * dd[e] - (e < 3 ? r2 : 0.0f);
-                acc[g][e] += (double)((-t * zl + dd[e] * zr) * damp);
+                for (int e = 0; e < 6; ++e) {
+                  const float t = 3.0f * dd[e] - (e < 3 ? r2 : 0.0f);
+                  acc[g][e] += (double)((-t * zl + dd[e] * zr) * damp);
+                }
               }
             }
           }
@@ -365,10 +406,13 @@ __global__ __launch_bounds__(256) void k_cfdm_build(
         d0y = (xyz[3 * p + 1] - xyz[3 * q + 1]) + dcy;
         d0z = (xyz[3 * p + 2] - xyz[3 * q + 2]) + dcz;
       }
-      for (int s = 0; s < nimg; ++s) {
-        double dx, dy, dz, r;
-        add3(d0x, d0y, d0z, img[3 * s], img[3 * s + 1], img[3 * s + 2], dx, dy, dz);
-        if (in_sphere(dx, dy, dz, rc, rc2x, r)) lr_tensor_add64(dx, dy, dz, r, R12, beta, pref, acc);
+      for (int c0 = 0; c0 < nimg; c0 += kImgChunk) {
+        unsigned m = sphere_mask(d0x, d0y, d0z, img, c0, min(kImgChunk, nimg - c0), -1, rc2x);
+        while (m) {
+          double dx, dy, dz, r;
+          if (take_marked(d0x, d0y, d0z, img, c0, m, rc, rc2x, dx, dy, dz, r))
+            lr_tensor_add64(dx, dy, dz, r, R12, beta, pref, acc);
+        }
       }
     }
     put_block(S, lx, ly, acc);
