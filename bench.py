@@ -301,6 +301,11 @@ def main():
                 others[k]["traffic_basis"] = (
                     f"counter ratio {r} (FETCH_SIZE+WRITE_SIZE over algorithmic bytes, measured on "
                     f"the same workload) times the algorithmic bytes per launch of this run")
+    for k in ("scs_build", "cfdm_build"):
+        if k in others:
+            others[k]["note"] = ("bound by the vector ALU (exp, erf and a division per image term "
+                                 "inside the cutoff sphere), not by HBM: the GB/s figure counts "
+                                 "the output blocks written once")
     out["roofline_other"] = others
     eng.close()
     if not args.no_batch:
