@@ -39,9 +39,9 @@ def test_more_than_one_gpu_exits_with_message():
 def test_traffic_ratio_read_from_profiles():
     sys.path.insert(0, ROOT)
     import bench
-    assert abs(bench.measured_traffic_ratio("sbr_chase") - 0.3322) < 1e-12
+    assert abs(bench.measured_traffic_ratio("sbr_chase") - 0.3309) < 1e-12
     assert abs(bench.measured_traffic_ratio("sbr_symm") - 2.1711) < 1e-12
     assert abs(bench.measured_traffic_ratio("pcg_symv") - 1.1263) < 1e-12
-    assert abs(bench.measured_traffic_ratio("scs_build") - 1.0196) < 1e-12
+    assert abs(bench.measured_traffic_ratio("scs_build") - 1.0197) < 1e-12
     assert bench.measured_traffic_ratio("trd_symv") is None
     assert bench.measured_traffic_ratio("no_such_kernel") is None
